@@ -1,0 +1,19 @@
+"""cc3d for MI355X: connected-components labelling on the GPU.
+
+Drop-in for the labelling path of connected-components-3d 4.0.0
+(cc3d/__init__.py:6-17 re-exports; cc3d/fastcc3d.pyx:244-626): the same call
+signatures and results, computed by hand-written HIP kernels for gfx950
+(libcc3d_amd.so, C ABI in include/cc3d_amd.h).
+"""
+from ._core import DimensionError, connected_components, estimate_provisional_labels
+from ._follow import dust, statistics
+
+__version__ = "4.0.0+mi355x.0.1"
+
+__all__ = [
+  "connected_components",
+  "estimate_provisional_labels",
+  "statistics",
+  "dust",
+  "DimensionError",
+]

@@ -1,0 +1,1124 @@
+// Global phases of the labelling pipeline and the C ABI (include/cc3d_amd.h).
+//
+// Pipeline for one volume (stream-ordered, no host round trip until stats):
+//   k_init       reset counters, clear the root bitmask
+//   [k_minmax]   continuous 2D-8 only: global min/max (cc3d/cc3d_continuous.hpp:298-303)
+//   k_local      per tile: LDS union-find, local roots, J tables, requests
+//   k_requests   per tile: cross-tile unions on the global key forest GP
+//   [k_periodic] wrap-around unions (cc3d/cc3d.hpp:1048-1073, :1265-1277, :1377-1418)
+//   k_roots      per local root: find + compress; global roots set a bit at their key
+//   k_scan_*     exclusive popcount prefix over the bitmask -> N
+//   k_labels     per local root: final label = 1 + rank of its global root's key
+//   --- stage 2 ---
+//   k_finalize   per voxel: label of its local root, written in the output dtype
+//   [k_q1_*]     binary 2D-8 only: provisional count and stale background labels
+//
+// Numbering: a component's label is 1 + the number of components whose
+// minimum key is smaller -- the reference's first-appearance order
+// (cc3d/cc3d.hpp:239-285), keys being voxel indices (or 2x2 block keys for the
+// binary 2D-8 path, cc3d/cc3d_binary.hpp:1212-1231).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/cc3d_amd.h"
+#include "common.hpp"
+#include "local.hpp"
+
+namespace cc3d_amd {
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void k_init(Counters* c, uint32_t* bits, int64_t nwords) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    c->epl = 0;
+    c->prov = 0;
+    c->first_row = 0x7FFFFFFFFFFFFFFFll;
+    c->last_row = -1;
+    c->req_count = 0;
+    c->req_overflow = 0;
+    c->N = 0;
+  }
+  for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
+}
+
+template <class G>
+struct TileIdx {
+  int64_t ntx, nty, ntz;
+  __device__ inline void coords(int64_t tile, int64_t& tx, int64_t& ty, int64_t& tz) const {
+    tz = tile / (ntx * nty);
+    const int64_t r = tile - tz * ntx * nty;
+    ty = r / ntx;
+    tx = r - ty * ntx;
+  }
+};
+
+struct GlobalParams {
+  int64_t sx, sy, sz, osx;
+  int64_t ntx, nty, ntz, ntiles;
+  uint32_t* GP;
+  uint32_t* RL;
+  const uint32_t* TR;
+  const uint8_t* J8;
+  const uint16_t* J16;
+  const uint32_t* REQ;
+  const uint32_t* TQ;
+  uint32_t* bits;
+  uint32_t* prefix;
+  uint32_t* bsum;
+  int64_t nwords;
+  Counters* ctr;
+};
+
+// Key of the local root of voxel (gx, gy, gz); 0xFFFFFFFF for background.
+template <class G>
+__device__ inline uint32_t lroot_key(const GlobalParams& p, int64_t gx, int64_t gy, int64_t gz) {
+  const int64_t tx = gx / G::TX, ty = gy / G::TY, tz = G::IS3D ? gz / G::TZ : 0;
+  const int64_t tile = tx + p.ntx * (ty + p.nty * tz);
+  const int pos = (int)(gx - tx * G::TX) + G::TX * ((int)(gy - ty * G::TY) + G::TY * (int)(gz - tz * G::TZ));
+  const uint32_t R = p.TR[tile];
+  const uint32_t j = R <= 255 ? p.J8[tile * G::TV + pos] : p.J16[tile * G::TV + pos];
+  if (j == 0) return 0xFFFFFFFFu;
+  return p.RL[tile * G::TV + j - 1];
+}
+
+// Cross-tile unions: one wave per tile walks that tile's requests.
+template <class G, int QK>
+__global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
+  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
+    const uint32_t nq = p.TQ[tile];
+    if (nq == 0) continue;
+    const int64_t tz = tile / (p.ntx * p.nty);
+    const int64_t tr = tile - tz * p.ntx * p.nty;
+    const int64_t ty = tr / p.ntx, tx = tr - ty * p.ntx;
+    const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
+    for (uint32_t i = lane; i < nq; i += 64) {
+      const uint32_t rq = p.REQ[tile * G::HALO + i];
+      const int ps = rq >> 16, h = rq & 0xFFFF;
+      const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
+      const uint32_t a = gkey<QK>(x0 + lx, y0 + ly, z0 + lz, p.sx, p.sy, p.osx);
+      int bx, by, bz;
+      G::halo_coords(h, bx, by, bz);
+      const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
+      const uint32_t b = lroot_key<G>(p, gx, gy, gz);
+      if (b != 0xFFFFFFFFu) gunion(p.GP, a, b);
+    }
+  }
+}
+
+// Periodic boundary unions.  NB: 6 (x, y, z wraps), 4 (x, y wraps) or 8 (y wrap
+// with diagonals, the two cross corners, x wrap with diagonals), matching the
+// pairs the reference visits.  Binary links need both voxels set; multilabel
+// links need equal nonzero values.  Skipped when the reference would have taken
+// its single-foreground-row shortcut (cc3d/fastcc3d.pyx:469-470), which ignores
+// periodicity.
+template <class T, int MODE, int NB, class G>
+__global__ __launch_bounds__(WG) void k_periodic(GlobalParams p, const T* in, int special_row_rule) {
+  if (special_row_rule && p.ctr->first_row == p.ctr->last_row && p.ctr->first_row >= 0) return;
+  const int64_t sx = p.sx, sy = p.sy, sz = p.sz;
+  int64_t n0, n1, n2;
+  if (NB == 6) {
+    n0 = sy * sz;
+    n1 = sx * sz;
+    n2 = sx * sy;
+  } else if (NB == 4) {
+    n0 = sx;
+    n1 = sy;
+    n2 = 0;
+  } else {
+    n0 = 3 * sx;
+    n1 = 2;
+    n2 = 3 * sy;
+  }
+  const int64_t total = n0 + n1 + n2;
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < total; i += (int64_t)gridDim.x * WG) {
+    int64_t ax, ay, az = 0, bx, by, bz = 0;
+    bool valid = true;
+    if (NB == 6) {
+      if (i < n0) {
+        ay = i % sy;
+        az = i / sy;
+        ax = 0;
+        bx = sx - 1;
+        by = ay;
+        bz = az;
+      } else if (i < n0 + n1) {
+        const int64_t k = i - n0;
+        ax = k % sx;
+        az = k / sx;
+        ay = 0;
+        bx = ax;
+        by = sy - 1;
+        bz = az;
+      } else {
+        const int64_t k = i - n0 - n1;
+        ax = k % sx;
+        ay = k / sx;
+        az = 0;
+        bx = ax;
+        by = ay;
+        bz = sz - 1;
+      }
+    } else if (NB == 4) {
+      if (i < n0) {
+        ax = i;
+        ay = 0;
+        bx = ax;
+        by = sy - 1;
+      } else {
+        ay = i - n0;
+        ax = 0;
+        bx = sx - 1;
+        by = ay;
+      }
+    } else {
+      if (i < n0) {
+        const int64_t x = i / 3, d = i % 3 - 1;
+        ax = x;
+        ay = 0;
+        bx = x + d;
+        by = sy - 1;
+        valid = bx >= 0 && bx < sx;
+      } else if (i < n0 + n1) {
+        if (i == n0) {
+          ax = 0;
+          ay = 0;
+          bx = sx - 1;
+          by = sy - 1;
+        } else {
+          ax = sx - 1;
+          ay = 0;
+          bx = 0;
+          by = sy - 1;
+        }
+      } else {
+        const int64_t k = i - n0 - n1;
+        const int64_t y = k / 3, d = k % 3 - 1;
+        ax = 0;
+        ay = y;
+        bx = sx - 1;
+        by = y + d;
+        valid = by >= 0 && by < sy;
+      }
+    }
+    if (!valid) continue;
+    const T va = in[ax + sx * (ay + sy * az)];
+    const T vb = in[bx + sx * (by + sy * bz)];
+    if (!fg(va)) continue;
+    const bool link = MODE == M_BINARY ? fg(vb) : (va == vb);
+    if (!link) continue;
+    const uint32_t ka = lroot_key<G>(p, ax, ay, az), kb = lroot_key<G>(p, bx, by, bz);
+    if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) gunion(p.GP, ka, kb);
+  }
+}
+
+// Global roots: compress every local root onto its root; roots mark their key.
+template <class G>
+__global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
+  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
+    const uint32_t R = p.TR[tile];
+    for (uint32_t j = lane; j < R; j += 64) {
+      const uint32_t g = p.RL[tile * G::TV + j];
+      const uint32_t r = gfind(p.GP, g);
+      if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
+      else p.GP[g] = r;
+    }
+  }
+}
+
+// Exclusive popcount prefix over the root bitmask, 4096 words per block.
+constexpr int SCAN_WPT = 16;
+constexpr int SCAN_WPB = SCAN_WPT * WG;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t& total) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[wv] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int w = 0; w < WG / 64; w++) {
+    if (w < wv) base += tmp[w];
+    tot += tmp[w];
+  }
+  __syncthreads();
+  total = tot;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_a(const uint32_t* bits, int64_t nwords, uint32_t* bsum) {
+  __shared__ uint32_t tmp[WG / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_WPB + (int64_t)threadIdx.x * SCAN_WPT;
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_WPT; i++)
+    if (base + i < nwords) c += __popc(bits[base + i]);
+  uint32_t total;
+  block_exclusive_scan(c, tmp, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_b(uint32_t* bsum, int64_t nblk, Counters* ctr) {
+  __shared__ uint32_t tmp[WG / 64];
+  uint32_t carry = 0;
+  for (int64_t off = 0; off < nblk; off += WG) {
+    const int64_t i = off + threadIdx.x;
+    const uint32_t v = i < nblk ? bsum[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, tmp, total);
+    if (i < nblk) bsum[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0 && ctr) ctr->N = carry;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwords, const uint32_t* bsum,
+                                               uint32_t* prefix) {
+  __shared__ uint32_t tmp[WG / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_WPB + (int64_t)threadIdx.x * SCAN_WPT;
+  uint32_t w[SCAN_WPT];
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_WPT; i++) {
+    w[i] = base + i < nwords ? __popc(bits[base + i]) : 0u;
+    c += w[i];
+  }
+  uint32_t total;
+  uint32_t run = bsum[blockIdx.x] + block_exclusive_scan(c, tmp, total);
+#pragma unroll
+  for (int i = 0; i < SCAN_WPT; i++) {
+    if (base + i < nwords) prefix[base + i] = run;
+    run += w[i];
+  }
+}
+
+// Final label of every local root, written over its RL entry.
+template <class G>
+__global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
+  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
+    const uint32_t R = p.TR[tile];
+    for (uint32_t j = lane; j < R; j += 64) {
+      const uint32_t g = p.RL[tile * G::TV + j];
+      const uint32_t r = p.GP[g];
+      const uint32_t wd = r >> 5;
+      const uint32_t below = p.bits[wd] & ((1u << (r & 31)) - 1u);
+      p.RL[tile * G::TV + j] = p.prefix[wd] + __popc(below) + 1u;
+    }
+  }
+}
+
+// Stage 2: every voxel of the tile gets its local root's label.
+template <class OUT, class G>
+__global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* out) {
+  const int t = threadIdx.x;
+  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
+  const int64_t tz = tile / (p.ntx * p.nty);
+  const int64_t tr = tile - tz * p.ntx * p.nty;
+  const int64_t ty = tr / p.ntx, tx = tr - ty * p.ntx;
+  const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
+  const bool narrow = p.TR[tile] <= 255;
+  const uint32_t* LT = p.RL + tile * G::TV;
+#pragma unroll
+  for (int k = 0; k < G::PER_THREAD; k++) {
+    const int ps = t + WG * k;
+    const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
+    const int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (gx >= p.sx || gy >= p.sy || gz >= p.sz) continue;
+    const uint32_t j = narrow ? p.J8[tile * G::TV + ps] : p.J16[tile * G::TV + ps];
+    out[gx + p.sx * (gy + p.sy * gz)] = j ? (OUT)LT[j - 1] : (OUT)0;
+  }
+}
+
+// estimate_provisional_label_count (cc3d/cc3d.hpp:287-315): one wave per row.
+template <class T>
+__global__ __launch_bounds__(WG) void k_epl(const T* in, int64_t sx, int64_t rows, Counters* c) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
+  unsigned long long tot = 0;
+  long long first = 0x7FFFFFFFFFFFFFFFll, last = -1;
+  for (int64_t row = wave; row < rows; row += nwaves) {
+    const T* r = in + row * sx;
+    unsigned long long cnt = 0;
+    for (int64_t x = lane; x < sx; x += 64) {
+      const T v = r[x];
+      if (fg(v) && (x == 0 || v != r[x - 1])) cnt++;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    tot += cnt;
+    if (cnt) {
+      first = row < first ? row : first;
+      last = row > last ? row : last;
+    }
+  }
+  if (lane == 0) {
+    if (tot) atomicAdd(&c->epl, tot);
+    if (last >= 0) {
+      atomicMin(&c->first_row, first);
+      atomicMax(&c->last_row, last);
+    }
+  }
+}
+
+// Global min/max with the reference's sequential fold semantics
+// (cc3d/cc3d_continuous.hpp:298-303): std::max(a, b) = a < b ? b : a folds a
+// NaN in and lets the next value replace it, so the result is the plain
+// min/max of the elements after the last NaN (NaN if the last element is NaN).
+template <class T>
+__global__ __launch_bounds__(WG) void k_lastnan(const T* in, int64_t n, long long* lastnan) {
+  long long m = -1;
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
+    if (in[i] != in[i]) m = i;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const long long y = __shfl_xor(m, o);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m >= 0) atomicMax(lastnan, m);
+}
+
+template <class T>
+__device__ inline uint64_t ordkey(T v) {
+  if constexpr (sizeof(T) == 4 && T(-1) < T(0)) {
+    uint32_t b;
+    __builtin_memcpy(&b, &v, 4);
+    return (b & 0x80000000u) ? (uint64_t)(~b) : (uint64_t)(b | 0x80000000u);
+  } else if constexpr (sizeof(T) == 8 && T(-1) < T(0)) {
+    uint64_t b;
+    __builtin_memcpy(&b, &v, 8);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+  } else {
+    return (uint64_t)v;
+  }
+}
+template <class T>
+__host__ __device__ inline uint64_t unordkey(uint64_t k) {
+  if constexpr (sizeof(T) == 4 && T(-1) < T(0)) {
+    const uint32_t b = (k & 0x80000000u) ? (uint32_t)(k & 0x7FFFFFFFu) : (uint32_t)(~k);
+    return b;
+  } else if constexpr (sizeof(T) == 8 && T(-1) < T(0)) {
+    return (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  } else {
+    return k;
+  }
+}
+
+// mm[0] = min key, mm[1] = max key over indices > *lastnan
+template <class T>
+__global__ __launch_bounds__(WG) void k_minmax(const T* in, int64_t n, const long long* lastnan,
+                                               unsigned long long* mm) {
+  const int64_t start = *lastnan + 1;
+  unsigned long long lo = ~0ull, hi = 0;
+  for (int64_t i = start + (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) {
+    const unsigned long long k = ordkey(in[i]);
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(mm, lo);
+    atomicMax(mm + 1, hi);
+  }
+}
+
+template <class T>
+__global__ void k_minmax_final(const long long* lastnan, int64_t n, unsigned long long* mm) {
+  // converts ordered keys back to bit patterns; a trailing NaN makes both NaN
+  uint64_t lo, hi;
+  if (*lastnan == n - 1) {
+    lo = hi = (sizeof(T) == 4) ? 0x7FC00000ull : 0x7FF8000000000000ull;
+  } else {
+    lo = unordkey<T>(mm[0]);
+    hi = unordkey<T>(mm[1]);
+  }
+  mm[0] = lo;
+  mm[1] = hi;
+}
+
+// ---------------------------------------------------------------------------
+// binary 2D-8 (2x2 block) provisional labels: cc3d/cc3d_binary.hpp:1029-1161
+// ---------------------------------------------------------------------------
+// Source block of the block whose top-left pixel is (x, y): the block index
+// the reference copies its provisional label from, -1 for a new label, -2 for
+// a block without foreground.
+template <class T>
+__device__ inline int64_t q1_source(const T* in, int64_t x, int64_t y, int64_t sx, int64_t sy) {
+  const int64_t osx = (sx + 1) >> 1;
+  const int64_t ob = (x >> 1) + osx * (y >> 1);
+  auto px = [&](int64_t xx, int64_t yy) -> bool { return fg(in[xx + sx * yy]); };
+  const bool Y = px(x, y);
+  const bool Z = x < sx - 1 && px(x + 1, y);
+  const bool X = y < sy - 1 && px(x, y + 1);
+  const bool W = x < sx - 1 && y < sy - 1 && px(x + 1, y + 1);
+  const int64_t up = ob - osx, left = ob - 1, upleft = ob - 1 - osx, upright = ob + 1 - osx;
+  if (Y) {
+    if (y > 0 && px(x, y - 1)) return up;
+    if (y > 0 && x < sx - 1 && px(x + 1, y - 1)) return up;
+    if (x > 0 && px(x - 1, y)) return left;
+    if (x > 0 && y > 0 && px(x - 1, y - 1)) return upleft;
+    if (x > 0 && y < sy - 1 && px(x - 1, y + 1)) return left;
+    if (x < sx - 2 && y > 0 && Z && px(x + 2, y - 1)) return upright;
+    return -1;
+  }
+  if (Z) {
+    if (y > 0 && px(x + 1, y - 1)) return up;
+    if (y > 0 && px(x, y - 1)) return up;
+    if (x < sx - 2 && y > 0 && px(x + 2, y - 1)) return upright;
+    if (X) {
+      if (x > 0 && px(x - 1, y)) return left;
+      if (x > 0 && px(x - 1, y + 1)) return left;
+    }
+    return -1;
+  }
+  if (X) {
+    if (x > 0 && px(x - 1, y)) return left;
+    if (x > 0 && px(x - 1, y + 1)) return left;
+    return -1;
+  }
+  return W ? -1 : -2;
+}
+
+// Marks blocks taking a new provisional label in `bits` and counts them.
+template <class T>
+__global__ __launch_bounds__(WG) void k_q1_newblocks(const T* in, int64_t sx, int64_t sy, uint32_t* bits,
+                                                     Counters* c) {
+  const int64_t osx = (sx + 1) >> 1, osy = (sy + 1) >> 1, nb = osx * osy;
+  unsigned long long cnt = 0;
+  for (int64_t b = (int64_t)blockIdx.x * WG + threadIdx.x; b < nb; b += (int64_t)gridDim.x * WG) {
+    const int64_t by = b / osx, bx = b - by * osx;
+    if (q1_source(in, bx << 1, by << 1, sx, sy) == -1) {
+      atomicOr(bits + (b >> 5), 1u << (b & 31));
+      cnt++;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&c->prov, cnt);
+}
+
+// Stale background labels of the reference's in-place block expansion
+// (cc3d/cc3d_binary.hpp:1233-1260): rows with y*sx >= voxels/4 are rewritten
+// only on [xstart, xend) of rows y and y+1; pixels left of xstart or right of
+// xend whose index lies inside the block scratch (< osx*osy) keep the
+// provisional label of the block with that index.
+template <class T, class OUT>
+__global__ __launch_bounds__(WG) void k_q1_stale(const T* in, int64_t sx, int64_t sy, const uint32_t* bits,
+                                                 const uint32_t* prefix, OUT* out) {
+  const int64_t osx = (sx + 1) >> 1, osy = (sy + 1) >> 1, nb = osx * osy;
+  const int64_t V = sx * sy;
+  const int64_t target = V >> 2;
+  const int64_t ystart = (target + sx - 1) / sx;  // first row with y*sx >= target
+  const int64_t lo = ystart * sx;
+  for (int64_t loc = lo + (int64_t)blockIdx.x * WG + threadIdx.x; loc < nb; loc += (int64_t)gridDim.x * WG) {
+    const int64_t y = loc / sx, x = loc - y * sx;
+    const int64_t ymax = y < sy - 1 ? y + 1 : y;
+    // xstart/xend of compute_foreground_index (cc3d/cc3d.hpp:317-342) for y and ymax
+    int64_t s0 = 0, e0 = 0, s1 = 0, e1 = 0;
+    for (int64_t xx = 0; xx < sx; xx++)
+      if (fg(in[xx + sx * y])) { s0 = xx; break; }
+    for (int64_t xx = sx - 1; xx >= s0; xx--)
+      if (fg(in[xx + sx * y])) { e0 = xx + 1; break; }
+    for (int64_t xx = 0; xx < sx; xx++)
+      if (fg(in[xx + sx * ymax])) { s1 = xx; break; }
+    for (int64_t xx = sx - 1; xx >= s1; xx--)
+      if (fg(in[xx + sx * ymax])) { e1 = xx + 1; break; }
+    const int64_t xs = (s0 < s1 ? s0 : s1) & ~(int64_t)1;
+    const int64_t xe = e0 > e1 ? e0 : e1;
+    if (x >= xs && x < xe) continue;
+    // provisional label of block `loc`: follow copies back to a new-label block
+    const int64_t bby = loc / osx, bbx = loc - bby * osx;
+    int64_t src = q1_source(in, bbx << 1, bby << 1, sx, sy);
+    if (src == -2) continue;  // no foreground: the scratch held 0
+    int64_t b = loc;
+    while (src >= 0) {
+      b = src;
+      const int64_t yy = b / osx, xx = b - yy * osx;
+      src = q1_source(in, xx << 1, yy << 1, sx, sy);
+    }
+    const uint32_t w = (uint32_t)(b >> 5);
+    const uint32_t lab = prefix[w] + __popc(bits[w] & ((1u << (b & 31)) - 1u)) + 1u;
+    out[loc] = (OUT)lab;
+  }
+}
+
+__global__ void k_zero_words(uint32_t* w, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = 0;
+}
+
+template <class OUT>
+__global__ void k_zero_out(OUT* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = OUT(0);
+}
+
+__global__ void k_mm_init(unsigned long long* mm) {
+  mm[0] = ~0ull;
+  mm[1] = 0ull;
+  reinterpret_cast<long long*>(mm)[2] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                                                  \
+  do {                                                                                                  \
+    hipError_t _e = (expr);                                                                             \
+    if (_e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+static inline unsigned grid_for(int64_t items, int64_t per_block, int64_t cap) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, cap));
+}
+
+struct Geometry {
+  int64_t sx = 0, sy = 0, sz = 0, V = 0, K = 0, osx = 0;
+  int nb = 0, qk = Q_NONE, mode = M_MULTI;
+  bool is3d = true;
+  int64_t ntx = 0, nty = 0, ntz = 0, ntiles = 0;
+  int TV = 0, HALO = 0;
+  int64_t nwords = 0, nblk = 0;
+  int64_t qblocks = 0, qwords = 0, qnblk = 0;  // binary 2D-8 block bitmask
+  size_t oGP, oRL, oTR, oTQ, oJ8, oJ16, oREQ, oBITS, oPRE, oBSUM, oCTR, oMM, oQB, oQP, oQS, total;
+};
+
+static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary,
+                int continuous) {
+  if (sx <= 0 || sy <= 0 || sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
+  if (connectivity != 26 && connectivity != 18 && connectivity != 6 && connectivity != 8 && connectivity != 4)
+    return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
+  if ((connectivity == 8 || connectivity == 4) && sz != 1)
+    return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
+  g.sx = sx;
+  g.sy = sy;
+  g.sz = sz;
+  g.V = sx * sy * sz;
+  g.mode = binary ? M_BINARY : (continuous ? M_CONT : M_MULTI);
+  g.osx = (sx + 1) >> 1;
+  g.is3d = sz > 1;
+  g.qk = Q_NONE;
+  if (g.is3d) {
+    g.nb = (int)connectivity;
+  } else {
+    // a single plane: 26/18 reduce to the 8-neighbourhood, 6 to the 4-neighbourhood
+    g.nb = (connectivity == 26 || connectivity == 18) ? 8 : connectivity == 6 ? 4 : (int)connectivity;
+    if (connectivity == 8 && g.mode == M_BINARY) g.qk = Q_BLOCK8;
+    if (connectivity == 8 && g.mode == M_CONT) g.qk = Q_CONT8;
+    if (connectivity == 4 && g.mode == M_BINARY) g.qk = Q_BIN4;
+  }
+  g.K = g.qk == Q_BLOCK8 ? 4 * g.osx * ((sy + 1) >> 1) : g.V;
+  if (g.K >= 0xFFFFFFFFll || g.V >= 0xFFFFFFFFll)
+    return fail(CC3D_AMD_ERR_INVALID, "volumes of 2^32-1 voxels or more are not supported");
+  int TX, TY, TZ;
+  if (g.is3d) {
+    TX = Geo3::TX; TY = Geo3::TY; TZ = Geo3::TZ; g.TV = Geo3::TV; g.HALO = Geo3::HALO;
+  } else {
+    TX = Geo2::TX; TY = Geo2::TY; TZ = 1; g.TV = Geo2::TV; g.HALO = Geo2::HALO;
+  }
+  g.ntx = (sx + TX - 1) / TX;
+  g.nty = (sy + TY - 1) / TY;
+  g.ntz = (sz + TZ - 1) / TZ;
+  g.ntiles = g.ntx * g.nty * g.ntz;
+  g.nwords = (g.K + 31) / 32;
+  g.nblk = (g.nwords + SCAN_WPB - 1) / SCAN_WPB;
+  if (g.qk == Q_BLOCK8) {
+    g.qblocks = g.osx * ((sy + 1) >> 1);
+    g.qwords = (g.qblocks + 31) / 32;
+    g.qnblk = (g.qwords + SCAN_WPB - 1) / SCAN_WPB;
+  }
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += align256(bytes);
+    return r;
+  };
+  const size_t slots = (size_t)g.ntiles * g.TV;
+  g.oGP = take((size_t)g.K * 4);
+  g.oRL = take(slots * 4);
+  g.oTR = take((size_t)g.ntiles * 4);
+  g.oTQ = take((size_t)g.ntiles * 4);
+  g.oJ8 = take(slots);
+  g.oJ16 = take(slots * 2);
+  g.oREQ = take((size_t)g.ntiles * g.HALO * 4);
+  g.oBITS = take((size_t)g.nwords * 4);
+  g.oPRE = take((size_t)g.nwords * 4);
+  g.oBSUM = take((size_t)(g.nblk + 1) * 4);
+  g.oCTR = take(sizeof(Counters));
+  g.oMM = take(64);
+  g.oQB = take((size_t)g.qwords * 4);
+  g.oQP = take((size_t)g.qwords * 4);
+  g.oQS = take((size_t)(g.qnblk + 1) * 4);
+  g.total = o;
+  return CC3D_AMD_OK;
+}
+
+}  // namespace cc3d_amd
+
+using namespace cc3d_amd;
+
+struct cc3d_amd_job {
+  Geometry g;
+  int dt = 0;
+  uint64_t delta_bits = 0;
+  const void* in = nullptr;
+  hipStream_t stream = nullptr;
+  char* ws = nullptr;
+  bool own_ws = false;
+  int periodic = 0;
+  int special_row_rule = 0;
+  bool count_prov = false;  // binary 2D-8: count provisional block labels
+  bool stale = false;       // binary 2D-8 with an odd side: stale background labels
+  uint32_t* qbits = nullptr;
+  uint32_t* qprefix = nullptr;
+  uint32_t* qbsum = nullptr;
+  Counters* h_ctr = nullptr;
+  bool have_stats = false;
+  GlobalParams gp{};
+};
+
+namespace {
+
+template <class G>
+int run_global(cc3d_amd_job* j) {
+  const GlobalParams& p = j->gp;
+  hipStream_t s = j->stream;
+  const unsigned wave_blocks = grid_for(j->g.ntiles, 4, 65536);
+  if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  else hipLaunchKernelGGL((k_requests<G, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  HIPCHECK(hipGetLastError());
+  if (j->periodic && j->g.mode != M_CONT) {
+    // a 6-connected single plane wraps like the 4-neighbourhood (its z wrap pairs a voxel with itself)
+    const int nbp = j->g.is3d ? 6 : j->g.nb;
+    const int64_t npairs = nbp == 6 ? (j->g.sy * j->g.sz + j->g.sx * j->g.sz + j->g.sx * j->g.sy)
+                           : nbp == 4 ? (j->g.sx + j->g.sy) : (3 * j->g.sx + 2 + 3 * j->g.sy);
+    const dim3 grid(grid_for(npairs, WG, 65536));
+    const int sr = j->special_row_rule;
+#define CC3D_P(T)                                                                                             \
+  do {                                                                                                        \
+    const T* in = static_cast<const T*>(j->in);                                                               \
+    if (j->g.mode == M_BINARY) {                                                                              \
+      if (nbp == 6) hipLaunchKernelGGL((k_periodic<T, M_BINARY, 6, G>), grid, dim3(WG), 0, s, p, in, 0);     \
+      else if (nbp == 4) hipLaunchKernelGGL((k_periodic<T, M_BINARY, 4, G>), grid, dim3(WG), 0, s, p, in, 0); \
+      else hipLaunchKernelGGL((k_periodic<T, M_BINARY, 8, G>), grid, dim3(WG), 0, s, p, in, 0);              \
+    } else {                                                                                                  \
+      if (nbp == 6) hipLaunchKernelGGL((k_periodic<T, M_MULTI, 6, G>), grid, dim3(WG), 0, s, p, in, sr);     \
+      else if (nbp == 4) hipLaunchKernelGGL((k_periodic<T, M_MULTI, 4, G>), grid, dim3(WG), 0, s, p, in, sr); \
+      else hipLaunchKernelGGL((k_periodic<T, M_MULTI, 8, G>), grid, dim3(WG), 0, s, p, in, sr);              \
+    }                                                                                                         \
+  } while (0)
+    switch (j->dt) {
+      case DT_U8: CC3D_P(uint8_t); break;
+      case DT_U16: CC3D_P(uint16_t); break;
+      case DT_U32: CC3D_P(uint32_t); break;
+      case DT_U64: CC3D_P(uint64_t); break;
+      case DT_F32: CC3D_P(float); break;
+      case DT_F64: CC3D_P(double); break;
+    }
+#undef CC3D_P
+    HIPCHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL((k_roots<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_a, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
+  hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)j->g.nblk, p.ctr);
+  hipLaunchKernelGGL(k_scan_c, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
+  hipLaunchKernelGGL((k_labels<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+template <class T>
+int launch_minmax(cc3d_amd_job* j, unsigned long long* mm) {
+  hipStream_t s = j->stream;
+  const T* in = static_cast<const T*>(j->in);
+  long long* lastnan = reinterpret_cast<long long*>(mm + 2);
+  const dim3 grid(grid_for(j->g.V, WG, 4096));
+  hipLaunchKernelGGL(k_mm_init, dim3(1), dim3(1), 0, s, mm);
+  if (T(-1) < T(0)) hipLaunchKernelGGL((k_lastnan<T>), grid, dim3(WG), 0, s, in, j->g.V, lastnan);
+  hipLaunchKernelGGL((k_minmax<T>), grid, dim3(WG), 0, s, in, j->g.V, lastnan, mm);
+  hipLaunchKernelGGL((k_minmax_final<T>), dim3(1), dim3(1), 0, s, lastnan, j->g.V, mm);
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+template <class T>
+int launch_q1(cc3d_amd_job* j) {
+  const Geometry& g = j->g;
+  hipStream_t s = j->stream;
+  hipLaunchKernelGGL(k_zero_words, dim3(grid_for(g.qwords, WG, 8192)), dim3(WG), 0, s, j->qbits, g.qwords);
+  hipLaunchKernelGGL((k_q1_newblocks<T>), dim3(grid_for(g.qblocks, WG, 65536)), dim3(WG), 0, s,
+                     static_cast<const T*>(j->in), g.sx, g.sy, j->qbits, j->gp.ctr);
+  HIPCHECK(hipGetLastError());
+  if (j->stale) {
+    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)g.qnblk), dim3(WG), 0, s, j->qbits, g.qwords, j->qbsum);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, j->qbsum, g.qnblk, (Counters*)nullptr);
+    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)g.qnblk), dim3(WG), 0, s, j->qbits, g.qwords, j->qbsum, j->qprefix);
+    HIPCHECK(hipGetLastError());
+  }
+  return CC3D_AMD_OK;
+}
+
+template <class T, class OUT>
+int launch_stale(cc3d_amd_job* j, OUT* out) {
+  const Geometry& g = j->g;
+  const int64_t n = g.qblocks - ((g.V >> 2) / g.sx) * g.sx;
+  hipLaunchKernelGGL((k_q1_stale<T, OUT>), dim3(grid_for(std::max<int64_t>(n, 1), WG, 4096)), dim3(WG), 0, j->stream,
+                     static_cast<const T*>(j->in), g.sx, g.sy, j->qbits, j->qprefix, out);
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+int delta_bits_of(int dt, const void* delta, uint64_t& bits) {
+  bits = 0;
+  if (!delta) return CC3D_AMD_OK;
+  const size_t sz = dt == DT_U8 ? 1 : dt == DT_U16 ? 2 : (dt == DT_U32 || dt == DT_F32) ? 4 : 8;
+  std::memcpy(&bits, delta, sz);
+  return CC3D_AMD_OK;
+}
+
+bool delta_is_zero(int dt, uint64_t bits) {
+  if (dt == DT_F32) return from_bits<float>(bits) == 0.0f;
+  if (dt == DT_F64) return from_bits<double>(bits) == 0.0;
+  return bits == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cc3d_amd_last_error(void) { return g_err.c_str(); }
+const char* cc3d_amd_version(void) { return "cc3d_amd 0.1.0 (gfx950)"; }
+
+size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary_image,
+                                int flags) {
+  Geometry g;
+  if (plan(g, sx, sy, sz, connectivity, binary_image, (flags & CC3D_AMD_CONTINUOUS) != 0) != CC3D_AMD_OK) return 0;
+  return g.total;
+}
+
+int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, int64_t sx, int64_t sy,
+                         int64_t sz, int64_t connectivity, const void* delta, int flags, int periodic_boundary,
+                         int binary_image, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!out_job) return fail(CC3D_AMD_ERR_INVALID, "job pointer is NULL");
+  *out_job = nullptr;
+  if (in_dtype < DT_U8 || in_dtype > DT_F64) return fail(CC3D_AMD_ERR_INVALID, "unsupported input dtype");
+  if (!in) return fail(CC3D_AMD_ERR_INVALID, "input pointer is NULL");
+  const bool continuous = (flags & CC3D_AMD_CONTINUOUS) && !binary_image;
+  cc3d_amd_job* j = new cc3d_amd_job();
+  int rc = plan(j->g, sx, sy, sz, connectivity, binary_image, continuous);
+  if (rc != CC3D_AMD_OK) {
+    delete j;
+    return rc;
+  }
+  if (periodic_boundary && j->g.mode == M_CONT) {
+    delete j;
+    return fail(CC3D_AMD_ERR_RUNTIME, "periodic_boundary is not currently supported for continuous data.");
+  }
+  const Geometry& g = j->g;
+  j->dt = in_dtype;
+  j->in = in;
+  j->stream = static_cast<hipStream_t>(stream);
+  j->periodic = periodic_boundary && (connectivity == 6 || connectivity == 4 || connectivity == 8);
+  j->special_row_rule = (g.mode == M_MULTI) && !(flags & CC3D_AMD_NO_SINGLE_ROW_RULE);
+  j->count_prov = g.qk == Q_BLOCK8 && (flags & CC3D_AMD_COUNT_PROVISIONAL);
+  j->stale = g.qk == Q_BLOCK8 && ((g.sx & 1) || (g.sy & 1));
+  delta_bits_of(in_dtype, delta, j->delta_bits);
+  if (workspace) {
+    if (workspace_bytes < g.total) {
+      delete j;
+      return fail(CC3D_AMD_ERR_INVALID, "workspace too small");
+    }
+    j->ws = static_cast<char*>(workspace);
+  } else {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, g.total, j->stream) != hipSuccess || !p) {
+      delete j;
+      return fail(CC3D_AMD_ERR_NOMEM, "could not allocate workspace");
+    }
+    j->ws = static_cast<char*>(p);
+    j->own_ws = true;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&j->h_ctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+    cc3d_amd_job_free(j);
+    return fail(CC3D_AMD_ERR_NOMEM, "could not allocate pinned counters");
+  }
+  char* ws = j->ws;
+  GlobalParams& gp = j->gp;
+  gp.sx = g.sx; gp.sy = g.sy; gp.sz = g.sz; gp.osx = g.osx;
+  gp.ntx = g.ntx; gp.nty = g.nty; gp.ntz = g.ntz; gp.ntiles = g.ntiles;
+  gp.GP = reinterpret_cast<uint32_t*>(ws + g.oGP);
+  gp.RL = reinterpret_cast<uint32_t*>(ws + g.oRL);
+  gp.TR = reinterpret_cast<uint32_t*>(ws + g.oTR);
+  gp.J8 = reinterpret_cast<uint8_t*>(ws + g.oJ8);
+  gp.J16 = reinterpret_cast<uint16_t*>(ws + g.oJ16);
+  gp.REQ = reinterpret_cast<uint32_t*>(ws + g.oREQ);
+  gp.TQ = reinterpret_cast<uint32_t*>(ws + g.oTQ);
+  gp.bits = reinterpret_cast<uint32_t*>(ws + g.oBITS);
+  gp.prefix = reinterpret_cast<uint32_t*>(ws + g.oPRE);
+  gp.bsum = reinterpret_cast<uint32_t*>(ws + g.oBSUM);
+  gp.nwords = g.nwords;
+  gp.ctr = reinterpret_cast<Counters*>(ws + g.oCTR);
+  j->qbits = reinterpret_cast<uint32_t*>(ws + g.oQB);
+  j->qprefix = reinterpret_cast<uint32_t*>(ws + g.oQP);
+  j->qbsum = reinterpret_cast<uint32_t*>(ws + g.oQS);
+  unsigned long long* mm = reinterpret_cast<unsigned long long*>(ws + g.oMM);
+  hipStream_t s = j->stream;
+
+  hipLaunchKernelGGL(k_init, dim3(grid_for(g.nwords, WG, 8192)), dim3(WG), 0, s, gp.ctr, gp.bits, g.nwords);
+  if (hipGetLastError() != hipSuccess) {
+    cc3d_amd_job_free(j);
+    return fail(CC3D_AMD_ERR_HIP, "k_init launch failed");
+  }
+
+#define CC3D_DT_SWITCH(FN)                      \
+  switch (in_dtype) {                           \
+    case DT_U8: rc = FN<uint8_t>; break;        \
+    case DT_U16: rc = FN<uint16_t>; break;      \
+    case DT_U32: rc = FN<uint32_t>; break;      \
+    case DT_U64: rc = FN<uint64_t>; break;      \
+    case DT_F32: rc = FN<float>; break;         \
+    default: rc = FN<double>; break;            \
+  }
+  if (g.qk == Q_CONT8) {
+#define CC3D_MM launch_minmax
+    switch (in_dtype) {
+      case DT_U8: rc = launch_minmax<uint8_t>(j, mm); break;
+      case DT_U16: rc = launch_minmax<uint16_t>(j, mm); break;
+      case DT_U32: rc = launch_minmax<uint32_t>(j, mm); break;
+      case DT_U64: rc = launch_minmax<uint64_t>(j, mm); break;
+      case DT_F32: rc = launch_minmax<float>(j, mm); break;
+      default: rc = launch_minmax<double>(j, mm); break;
+    }
+#undef CC3D_MM
+    if (rc != CC3D_AMD_OK) {
+      cc3d_amd_job_free(j);
+      return rc;
+    }
+  }
+#undef CC3D_DT_SWITCH
+
+  LocalParams lp;
+  lp.in = in;
+  lp.sx = g.sx; lp.sy = g.sy; lp.sz = g.sz; lp.osx = g.osx;
+  lp.ntx = g.ntx; lp.nty = g.nty; lp.ntz = g.ntz; lp.ntiles = g.ntiles;
+  lp.delta_bits = j->delta_bits;
+  lp.minmax = reinterpret_cast<const uint64_t*>(mm);
+  lp.GP = gp.GP;
+  lp.RL = gp.RL;
+  lp.TR = reinterpret_cast<uint32_t*>(ws + g.oTR);
+  lp.J8 = reinterpret_cast<uint8_t*>(ws + g.oJ8);
+  lp.J16 = reinterpret_cast<uint16_t*>(ws + g.oJ16);
+  lp.REQ = reinterpret_cast<uint32_t*>(ws + g.oREQ);
+  lp.TQ = reinterpret_cast<uint32_t*>(ws + g.oTQ);
+  lp.ctr = gp.ctr;
+  hipError_t e;
+  if (g.mode == M_MULTI) e = launch_local_multi(in_dtype, g.nb, g.qk, lp, s);
+  else if (g.mode == M_BINARY) e = launch_local_binary(in_dtype, g.nb, g.qk, lp, s);
+  else e = launch_local_cont(in_dtype, g.nb, g.qk, lp, s);
+  if (e != hipSuccess) {
+    cc3d_amd_job_free(j);
+    return fail(CC3D_AMD_ERR_HIP, std::string("k_local: ") + hipGetErrorString(e));
+  }
+  rc = g.is3d ? run_global<Geo3>(j) : run_global<Geo2>(j);
+  if (rc == CC3D_AMD_OK && (j->count_prov || j->stale)) {
+    switch (in_dtype) {
+      case DT_U8: rc = launch_q1<uint8_t>(j); break;
+      case DT_U16: rc = launch_q1<uint16_t>(j); break;
+      case DT_U32: rc = launch_q1<uint32_t>(j); break;
+      case DT_U64: rc = launch_q1<uint64_t>(j); break;
+      case DT_F32: rc = launch_q1<float>(j); break;
+      default: rc = launch_q1<double>(j); break;
+    }
+  }
+  if (rc != CC3D_AMD_OK) {
+    cc3d_amd_job_free(j);
+    return rc;
+  }
+  *out_job = j;
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_job_stats(cc3d_amd_job* j, cc3d_amd_stats* st) {
+  if (!j || !st) return fail(CC3D_AMD_ERR_INVALID, "NULL job or stats");
+  if (!j->have_stats) {
+    HIPCHECK(hipMemcpyAsync(j->h_ctr, j->gp.ctr, sizeof(Counters), hipMemcpyDeviceToHost, j->stream));
+    HIPCHECK(hipStreamSynchronize(j->stream));
+    j->have_stats = true;
+  }
+  st->N = j->h_ctr->N;
+  st->provisional = j->h_ctr->prov;
+  st->epl = j->h_ctr->epl;
+  st->first_foreground_row = j->h_ctr->last_row < 0 ? -1 : j->h_ctr->first_row;
+  st->last_foreground_row = j->h_ctr->last_row;
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_label_finish(cc3d_amd_job* j, void* out, int out_dtype) {
+  if (!j || !out) return fail(CC3D_AMD_ERR_INVALID, "NULL job or output");
+  const Geometry& g = j->g;
+  hipStream_t s = j->stream;
+  const dim3 grid((unsigned)g.ntiles), block(WG);
+  int rc = CC3D_AMD_OK;
+#define CC3D_F(OUT)                                                                                \
+  do {                                                                                             \
+    if (g.is3d) hipLaunchKernelGGL((k_finalize<OUT, Geo3>), grid, block, 0, s, j->gp, (OUT*)out); \
+    else hipLaunchKernelGGL((k_finalize<OUT, Geo2>), grid, block, 0, s, j->gp, (OUT*)out);        \
+    if (j->stale) {                                                                                \
+      switch (j->dt) {                                                                             \
+        case DT_U8: rc = launch_stale<uint8_t, OUT>(j, (OUT*)out); break;                          \
+        case DT_U16: rc = launch_stale<uint16_t, OUT>(j, (OUT*)out); break;                        \
+        case DT_U32: rc = launch_stale<uint32_t, OUT>(j, (OUT*)out); break;                        \
+        case DT_U64: rc = launch_stale<uint64_t, OUT>(j, (OUT*)out); break;                        \
+        case DT_F32: rc = launch_stale<float, OUT>(j, (OUT*)out); break;                           \
+        default: rc = launch_stale<double, OUT>(j, (OUT*)out); break;                              \
+      }                                                                                            \
+    }                                                                                              \
+  } while (0)
+  switch (out_dtype) {
+    case DT_U16: CC3D_F(uint16_t); break;
+    case DT_U32: CC3D_F(uint32_t); break;
+    case DT_U64: CC3D_F(uint64_t); break;
+    default: return fail(CC3D_AMD_ERR_INVALID, "output dtype must be uint16, uint32 or uint64");
+  }
+#undef CC3D_F
+  if (rc != CC3D_AMD_OK) return rc;
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+void cc3d_amd_job_free(cc3d_amd_job* j) {
+  if (!j) return;
+  if (j->own_ws && j->ws) (void)hipFreeAsync(j->ws, j->stream);
+  if (j->h_ctr) (void)hipHostFree(j->h_ctr);
+  delete j;
+}
+
+int cc3d_amd_estimate_provisional_label_count(const void* in, int in_dtype, int64_t sx, int64_t voxels,
+                                              uint64_t* epl, int64_t* first_row, int64_t* last_row,
+                                              void* stream) {
+  if (!in || !epl || !first_row || !last_row) return fail(CC3D_AMD_ERR_INVALID, "NULL argument");
+  if (sx <= 0 || voxels <= 0 || voxels % sx) return fail(CC3D_AMD_ERR_INVALID, "bad geometry");
+  if (in_dtype < DT_U8 || in_dtype > DT_F64) return fail(CC3D_AMD_ERR_INVALID, "unsupported input dtype");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Counters* d = nullptr;
+  Counters* h = nullptr;
+  HIPCHECK(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(Counters), s));
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(WG), 0, s, d, (uint32_t*)nullptr, (int64_t)0);
+  const int64_t rows = voxels / sx;
+  const dim3 grid(grid_for(rows, 4, 16384));
+  switch (in_dtype) {
+    case DT_U8: hipLaunchKernelGGL((k_epl<uint8_t>), grid, dim3(WG), 0, s, (const uint8_t*)in, sx, rows, d); break;
+    case DT_U16: hipLaunchKernelGGL((k_epl<uint16_t>), grid, dim3(WG), 0, s, (const uint16_t*)in, sx, rows, d); break;
+    case DT_U32: hipLaunchKernelGGL((k_epl<uint32_t>), grid, dim3(WG), 0, s, (const uint32_t*)in, sx, rows, d); break;
+    case DT_U64: hipLaunchKernelGGL((k_epl<uint64_t>), grid, dim3(WG), 0, s, (const uint64_t*)in, sx, rows, d); break;
+    case DT_F32: hipLaunchKernelGGL((k_epl<float>), grid, dim3(WG), 0, s, (const float*)in, sx, rows, d); break;
+    default: hipLaunchKernelGGL((k_epl<double>), grid, dim3(WG), 0, s, (const double*)in, sx, rows, d); break;
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(Counters), hipHostMallocDefault));
+  HIPCHECK(hipMemcpyAsync(h, d, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  *epl = h->epl;
+  *first_row = h->last_row < 0 ? -1 : h->first_row;
+  *last_row = h->last_row;
+  (void)hipHostFree(h);
+  (void)hipFreeAsync(d, s);
+  return CC3D_AMD_OK;
+}
+
+// Union-find length the reference's first pass allocates for these arguments
+// (cc3d/cc3d.hpp:360-366 and its 18/6/2D twins; cc3d/cc3d_binary.hpp:47-51,
+// :350-352, :527-530, :794-797, :964-967; cc3d/cc3d_continuous.hpp:105-107,
+// :224-226, :289-291), or 0 when that pass returns before labelling.
+uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t connectivity, int binary_image,
+                                    int out_dtype) {
+  const uint64_t omax = out_dtype == DT_U16 ? 0xFFFFull : out_dtype == DT_U32 ? 0xFFFFFFFFull : ~0ull;
+  const uint64_t V = (uint64_t)voxels;
+  uint64_t ml = (uint64_t)max_labels + 1;
+  if (binary_image && connectivity == 8) {
+    ml = std::max<uint64_t>(std::min<uint64_t>(ml, (V >> 1) + 1), 1) + 1;
+    return std::min(ml, omax);
+  }
+  if (max_labels == 0) return 0;
+  if (binary_image && connectivity == 26) ml = std::min<uint64_t>(ml, (V >> 2) + 1) + 1;
+  else if (binary_image && (connectivity == 6 || connectivity == 4)) ml = std::min<uint64_t>(ml, (V >> 1) + 1) + 1;
+  else ml = std::min<uint64_t>(ml, V + 1);
+  return std::min(ml, omax);
+}
+
+int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, int64_t sy, int64_t sz,
+                                    int64_t max_labels, int64_t connectivity, const void* delta, void* out,
+                                    int out_dtype, uint64_t* N, int periodic_boundary, int binary_image,
+                                    void* stream) {
+  if (!out || !N) return fail(CC3D_AMD_ERR_INVALID, "NULL output");
+  if (out_dtype != DT_U16 && out_dtype != DT_U32 && out_dtype != DT_U64)
+    return fail(CC3D_AMD_ERR_INVALID, "output dtype must be uint16, uint32 or uint64");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint64_t db = 0;
+  delta_bits_of(in_dtype, delta, db);
+  const bool continuous = !binary_image && !delta_is_zero(in_dtype, db);
+  const int64_t V = sx * sy * sz;
+  *N = 0;
+  const uint64_t L = cc3d_amd_union_find_length(max_labels, V, connectivity, binary_image, out_dtype);
+  if (L == 0) {
+    // the reference returns before its first pass: no foreground is labelled
+    if (sx <= 0 || sy <= 0 || sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
+    const dim3 grid(grid_for(V, WG, 65536));
+    if (out_dtype == DT_U16) hipLaunchKernelGGL((k_zero_out<uint16_t>), grid, dim3(WG), 0, s, (uint16_t*)out, V);
+    else if (out_dtype == DT_U32) hipLaunchKernelGGL((k_zero_out<uint32_t>), grid, dim3(WG), 0, s, (uint32_t*)out, V);
+    else hipLaunchKernelGGL((k_zero_out<uint64_t>), grid, dim3(WG), 0, s, (uint64_t*)out, V);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    return CC3D_AMD_OK;
+  }
+  const bool b8 = binary_image && connectivity == 8 && sz == 1;
+  int flags = (continuous ? CC3D_AMD_CONTINUOUS : 0) | CC3D_AMD_NO_SINGLE_ROW_RULE;
+  if (b8 && (uint64_t)(((sx + 1) >> 1) * ((sy + 1) >> 1)) >= L) flags |= CC3D_AMD_COUNT_PROVISIONAL;
+  cc3d_amd_job* j = nullptr;
+  int rc = cc3d_amd_label_begin(&j, in, in_dtype, sx, sy, sz, connectivity, delta, flags, periodic_boundary,
+                                binary_image, nullptr, 0, stream);
+  if (rc != CC3D_AMD_OK) return rc;
+  cc3d_amd_stats st;
+  rc = cc3d_amd_job_stats(j, &st);
+  if (rc == CC3D_AMD_OK && st.provisional >= L && (!b8 || (flags & CC3D_AMD_COUNT_PROVISIONAL)))
+    rc = fail(CC3D_AMD_ERR_RUNTIME, "maximum length exception");
+  if (rc == CC3D_AMD_OK) rc = cc3d_amd_label_finish(j, out, out_dtype);
+  if (rc == CC3D_AMD_OK) *N = st.N;
+  cc3d_amd_job_free(j);
+  if (rc == CC3D_AMD_OK) HIPCHECK(hipStreamSynchronize(s));
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// Shared definitions for the cc3d MI355X labelling kernels.
+//
+// Layout conventions (all kernels):
+//   * volumes are dense, x fastest: index = x + sx * (y + sy * z), the layout
+//     cc3d/fastcc3d.pyx:352-359 establishes for both C and F inputs;
+//   * the volume is cut into tiles; one 256-thread workgroup owns one tile in
+//     the local kernel and the finalize kernel;
+//   * union-find "keys" order components for numbering: the voxel index, or for
+//     the binary 2D 8-connected path the 2x2 block key
+//     ((y>>1)*osx + (x>>1))*4 + (y&1)*2 + (x&1) (cc3d/cc3d_binary.hpp:1016-1161).
+//     The root of every component is its minimum key, which is the key of the
+//     voxel that receives the first provisional label in the reference scan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cc3d_amd {
+
+enum DType : int { DT_U8 = 0, DT_U16 = 1, DT_U32 = 2, DT_U64 = 3, DT_F32 = 4, DT_F64 = 5 };
+enum Mode : int { M_MULTI = 0, M_BINARY = 1, M_CONT = 2 };
+// Quirk variants of the 2D paths (see the kernel headers for the rules).
+enum Quirk : int { Q_NONE = 0, Q_BLOCK8 = 1, Q_BIN4 = 2, Q_CONT8 = 3 };
+
+constexpr int WG = 256;
+
+// Device-side counters of one labelling job.
+struct Counters {
+  unsigned long long epl;        // provisional-label estimate (transitions)
+  unsigned long long prov;       // voxels (or blocks) taking a new provisional label
+  long long first_row;           // first row holding foreground (INT64_MAX if none)
+  long long last_row;            // last row holding foreground (-1 if none)
+  unsigned int req_count;        // cross-tile union requests emitted
+  unsigned int req_overflow;     // 1 when the request buffer overflowed
+  unsigned long long N;          // number of components
+  unsigned long long pad[2];
+};
+
+// Tile geometry.  A tile is TX x TY x TZ voxels; the workgroup stages it in LDS
+// together with a one-voxel halo on the sides that backward neighbours reach
+// (x-1..x+1, y-1..y+1 and z-1 in 3D; x-1..x+1 and y-1 in 2D).
+template <int TX_, int TY_, int TZ_, bool IS3D_>
+struct TileGeo {
+  static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
+  static constexpr bool IS3D = IS3D_;
+  static constexpr int TV = TX * TY * TZ;
+  static constexpr int PER_THREAD = TV / WG;
+  static constexpr int BX = TX + 2;
+  static constexpr int BY = IS3D ? TY + 2 : TY + 1;
+  static constexpr int BZ = IS3D ? TZ + 1 : 1;
+  static constexpr int BOXN = BX * BY * BZ;
+  static constexpr int RING = 2 * BX + 2 * (BY - 2);
+  static constexpr int HALO = IS3D ? BX * BY + TZ * RING : BX + 2 * TY;
+  static constexpr int NN = TV + HALO;
+  static_assert(TV % WG == 0, "tile must be a multiple of the workgroup");
+  static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0 && (TZ & (TZ - 1)) == 0,
+                "tile sides are powers of two");
+
+  __device__ static inline int box(int bx, int by, int bz) { return bx + BX * (by + BY * bz); }
+  __device__ static inline bool is_tile(int bx, int by, int bz) {
+    return bx >= 1 && bx <= TX && by >= 1 && by <= TY && (!IS3D || bz >= 1);
+  }
+  __device__ static inline int halo_index(int bx, int by, int bz) {
+    if (IS3D) {
+      if (bz == 0) return bx + BX * by;
+      int r;
+      if (by == 0) r = bx;
+      else if (by == BY - 1) r = BX + bx;
+      else r = 2 * BX + 2 * (by - 1) + (bx == 0 ? 0 : 1);
+      return BX * BY + (bz - 1) * RING + r;
+    } else {
+      if (by == 0) return bx;
+      return BX + 2 * (by - 1) + (bx == 0 ? 0 : 1);
+    }
+  }
+  __device__ static inline void halo_coords(int h, int& bx, int& by, int& bz) {
+    if (IS3D) {
+      if (h < BX * BY) {
+        bz = 0;
+        by = h / BX;
+        bx = h - by * BX;
+        return;
+      }
+      h -= BX * BY;
+      bz = h / RING + 1;
+      int r = h - (bz - 1) * RING;
+      if (r < BX) {
+        by = 0;
+        bx = r;
+      } else if (r < 2 * BX) {
+        by = BY - 1;
+        bx = r - BX;
+      } else {
+        r -= 2 * BX;
+        by = (r >> 1) + 1;
+        bx = (r & 1) ? BX - 1 : 0;
+      }
+    } else {
+      bz = 0;
+      if (h < BX) {
+        by = 0;
+        bx = h;
+        return;
+      }
+      h -= BX;
+      by = (h >> 1) + 1;
+      bx = (h & 1) ? BX - 1 : 0;
+    }
+  }
+  // Position of a tile voxel inside its tile (raster order); also the index
+  // used for the per-voxel tables J8/J16.
+  __device__ static inline int pos(int lx, int ly, int lz) { return lx + TX * (ly + TY * lz); }
+  // Union-find node of a tile voxel: its raster position, or the 2x2 block
+  // order used by the binary 2D 8-connected numbering.
+  template <int QK>
+  __device__ static inline int node(int lx, int ly, int lz) {
+    if (QK == Q_BLOCK8)
+      return ((((ly >> 1) * (TX / 2) + (lx >> 1)) << 2) | ((ly & 1) << 1) | (lx & 1));
+    return pos(lx, ly, lz);
+  }
+  template <int QK>
+  __device__ static inline int node_box(int bx, int by, int bz) {
+    if (is_tile(bx, by, bz)) return node<QK>(bx - 1, by - 1, IS3D ? bz - 1 : 0);
+    return TV + halo_index(bx, by, bz);
+  }
+  template <int QK>
+  __device__ static inline void node_to_local(int n, int& lx, int& ly, int& lz) {
+    if (QK == Q_BLOCK8) {
+      const int blk = n >> 2;
+      const int by = blk / (TX / 2), bxx = blk - by * (TX / 2);
+      lx = (bxx << 1) | (n & 1);
+      ly = (by << 1) | ((n >> 1) & 1);
+      lz = 0;
+    } else {
+      lx = n & (TX - 1);
+      ly = (n / TX) & (TY - 1);
+      lz = n / (TX * TY);
+    }
+  }
+};
+
+using Geo3 = TileGeo<32, 8, 8, true>;
+using Geo2 = TileGeo<64, 32, 1, false>;
+
+// Global key of a voxel.
+template <int QK>
+__device__ __host__ inline uint32_t gkey(int64_t gx, int64_t gy, int64_t gz, int64_t sx,
+                                         int64_t sy, int64_t osx) {
+  if (QK == Q_BLOCK8)
+    return (uint32_t)(((((gy >> 1) * osx + (gx >> 1)) << 2)) | ((gy & 1) << 1) | (gx & 1));
+  return (uint32_t)(gx + sx * (gy + sy * gz));
+}
+
+// XCD-aware tile order: MI355X dispatches workgroups round-robin over 8 XCDs,
+// each with a private L2.  Give every XCD a contiguous range of tiles so the
+// halos a tile re-reads were recently touched by the same L2.
+__device__ inline int64_t xcd_tile(int64_t bid, int64_t ntiles) {
+  const int64_t q = ntiles >> 3, r = ntiles & 7;
+  const int64_t x = bid & 7, k = bid >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------------------
+// value helpers
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline bool fg(T v) {
+  return v != T(0);
+}
+
+// cc3d/cc3d_continuous.hpp:79-88: unsigned types compare the absolute
+// difference without underflow; float types use std::abs(a - b) <= delta
+// evaluated in T.
+template <class T>
+__device__ inline bool match(T a, T b, T delta) {
+  if constexpr (T(-1) > T(0)) {
+    const T d = a > b ? T(a - b) : T(b - a);
+    return d <= delta;
+  } else {
+    const T d = a - b;
+    return (d < T(0) ? -d : d) <= delta;
+  }
+}
+
+template <class T>
+__device__ __host__ inline T from_bits(uint64_t b) {
+  T v;
+  __builtin_memcpy(&v, &b, sizeof(T));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// union-find
+// ---------------------------------------------------------------------------
+// LDS: parents are node ids; the root of a tree is its minimum node id.
+__device__ inline uint32_t lfind(volatile uint32_t* par, uint32_t a) {
+  uint32_t p = par[a];
+  while (p != a) {
+    a = p;
+    p = par[a];
+  }
+  return a;
+}
+
+// Lock-free union: link the larger root under the smaller with atomicMin and
+// retry from the displaced parent when another lane got there first.
+__device__ inline void lunion(uint32_t* par, uint32_t a, uint32_t b) {
+  while (true) {
+    a = lfind(par, a);
+    b = lfind(par, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicMin(&par[a], b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// Global union-find over keys.  Loads bypass the non-coherent L1 so retries see
+// other workgroups' links.
+__device__ inline uint32_t gload(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t gfind(const uint32_t* GP, uint32_t a) {
+  uint32_t p = gload(GP + a);
+  while (p != a) {
+    a = p;
+    p = gload(GP + a);
+  }
+  return a;
+}
+__device__ inline void gunion(uint32_t* GP, uint32_t a, uint32_t b) {
+  while (true) {
+    a = gfind(GP, a);
+    b = gfind(GP, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicMin(GP + a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward neighbourhoods, in the order the covering rule visits them
+// ---------------------------------------------------------------------------
+struct Off {
+  int dx, dy, dz;
+};
+// 3D: E K M B H D F J L A C G I  (first 3: 6-conn, first 9: 18-conn, all: 26)
+__host__ __device__ constexpr Off off3(int i) {
+  switch (i) {
+    case 0: return {0, 0, -1};
+    case 1: return {0, -1, 0};
+    case 2: return {-1, 0, 0};
+    case 3: return {0, -1, -1};
+    case 4: return {0, 1, -1};
+    case 5: return {-1, 0, -1};
+    case 6: return {1, 0, -1};
+    case 7: return {-1, -1, 0};
+    case 8: return {1, -1, 0};
+    case 9: return {-1, -1, -1};
+    case 10: return {1, -1, -1};
+    case 11: return {-1, 1, -1};
+    default: return {1, 1, -1};
+  }
+}
+// 2D: K M J L  (first 2: 4-conn, all: 8-conn)
+__host__ __device__ constexpr Off off2(int i) {
+  switch (i) {
+    case 0: return {0, -1, 0};
+    case 1: return {-1, 0, 0};
+    case 2: return {-1, -1, 0};
+    default: return {1, -1, 0};
+  }
+}
+
+template <int NB>
+struct Nbhd {
+  static constexpr bool IS3D = (NB == 26 || NB == 18 || NB == 6);
+  static constexpr int N = NB == 26 ? 13 : NB == 18 ? 9 : NB == 6 ? 3 : NB == 8 ? 4 : 2;
+  __host__ __device__ static constexpr Off off(int i) { return IS3D ? off3(i) : off2(i); }
+  // neighbours j adjacent (under NB) to neighbour i
+  __host__ __device__ static constexpr uint32_t adj(int i) {
+    uint32_t m = 0;
+    for (int j = 0; j < N; j++) {
+      if (j == i) continue;
+      const Off a = off(i), b = off(j);
+      const int ddx = a.dx - b.dx, ddy = a.dy - b.dy, ddz = a.dz - b.dz;
+      const int ax = ddx < 0 ? -ddx : ddx, ay = ddy < 0 ? -ddy : ddy, az = ddz < 0 ? -ddz : ddz;
+      const int linf = ax > ay ? (ax > az ? ax : az) : (ay > az ? ay : az);
+      const int l1 = ax + ay + az;
+      bool ok = linf <= 1;
+      if (NB == 18) ok = ok && l1 <= 2;
+      if (NB == 6 || NB == 4) ok = ok && l1 <= 1;
+      if (ok) m |= 1u << j;
+    }
+    return m;
+  }
+};
+
+}  // namespace cc3d_amd

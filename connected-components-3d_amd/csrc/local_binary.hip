@@ -1,0 +1,8 @@
+// Instantiations of the local labelling kernel for the binary mode.
+#include "local.hpp"
+
+namespace cc3d_amd {
+hipError_t launch_local_binary(int dt, int nb, int qk, const LocalParams& p, hipStream_t s) {
+  return launch_local_mode<M_BINARY>(dt, nb, qk, p, s);
+}
+}  // namespace cc3d_amd

@@ -1,0 +1,139 @@
+/*
+ * cc3d_amd.h -- C ABI of the MI355X labelling library (libcc3d_amd.so).
+ *
+ * Drop-in boundary for the labelling path of connected-components-3d 4.0.0.
+ * Every pointer named `in`/`out` is a device (HBM) pointer; `stream` is a
+ * hipStream_t (NULL = the legacy default stream).  Volumes are dense with x
+ * fastest: index = x + sx * (y + sy * z), which is how the reference's
+ * binding hands buffers to its C++ core (cc3d/fastcc3d.pyx:352-359).
+ *
+ * dtype codes: 0 u8, 1 u16, 2 u32, 3 u64, 4 f32, 5 f64.  Signed integers,
+ * bool and float16 (delta == 0) are passed as the unsigned view of the same
+ * width, exactly as the reference binding does (cc3d/fastcc3d.pyx:471-613).
+ * Output dtypes: 1 u16, 2 u32, 3 u64.
+ *
+ * Status codes: 0 ok; 1 runtime error (the reference's std::runtime_error,
+ * e.g. "maximum length exception"); 2 invalid argument; 3 HIP error;
+ * 4 out of memory.  cc3d_amd_last_error() returns the calling thread's last
+ * message.
+ */
+#ifndef CC3D_AMD_H
+#define CC3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC3D_AMD_OK 0
+#define CC3D_AMD_ERR_RUNTIME 1
+#define CC3D_AMD_ERR_INVALID 2
+#define CC3D_AMD_ERR_HIP 3
+#define CC3D_AMD_ERR_NOMEM 4
+
+/* Replaces cc3d::connected_components3d<T,U>(T* in, sx, sy, sz, max_labels,
+ * connectivity, T delta, U* out, size_t& N, bool periodic_boundary,
+ * bool binary_image) -- cc3d/cc3d_continuous.hpp:394-455, bound by the
+ * reference at cc3d/fastcc3d.pyx:67-74.  `delta` points to one host value of
+ * the input type.  Every output voxel is written (the reference relies on a
+ * pre-zeroed buffer).  max_labels sizes the reference's union-find; the same
+ * overflow ("maximum length exception") is reported here as status 1. */
+int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, int64_t sy,
+                                    int64_t sz, int64_t max_labels, int64_t connectivity,
+                                    const void* delta, void* out, int out_dtype, uint64_t* N,
+                                    int periodic_boundary, int binary_image, void* stream);
+
+/* Replaces cc3d::estimate_provisional_label_count<T>(T* in, sx, voxels,
+ * int64& first_foreground_row, int64& last_foreground_row) --
+ * cc3d/cc3d.hpp:287-315, bound at cc3d/fastcc3d.pyx:60-65. */
+int cc3d_amd_estimate_provisional_label_count(const void* in, int in_dtype, int64_t sx,
+                                              int64_t voxels, uint64_t* epl,
+                                              int64_t* first_foreground_row,
+                                              int64_t* last_foreground_row, void* stream);
+
+/* ---- two-stage form used by the Python binding ----------------------------
+ * The binding layer (cc3d/fastcc3d.pyx:244-626) needs the provisional-label
+ * estimate before it picks the output dtype.  Stage 1 labels the volume and
+ * computes that estimate in the same pass; stage 2 writes labels in the
+ * chosen dtype.  No reference counterpart: these split the one reference call
+ * so the estimate costs no extra pass over the input. */
+typedef struct cc3d_amd_job cc3d_amd_job;
+
+typedef struct {
+  uint64_t N;                  /* number of components */
+  uint64_t provisional;        /* provisional labels the reference would create */
+  uint64_t epl;                /* estimate_provisional_label_count */
+  int64_t first_foreground_row;
+  int64_t last_foreground_row;
+} cc3d_amd_stats;
+
+/* flags for the two-stage form */
+#define CC3D_AMD_CONTINUOUS 1          /* |a-b| <= delta linking (the delta != 0 path) */
+#define CC3D_AMD_COUNT_PROVISIONAL 2   /* binary 2D-8: count provisional block labels */
+#define CC3D_AMD_NO_SINGLE_ROW_RULE 4  /* do not mimic the binding's single-row shortcut */
+
+/* Bytes of scratch a job of this geometry needs (0 on bad arguments). */
+size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity,
+                                int binary_image, int flags);
+
+/* Stage 1.  workspace may be NULL (the library allocates it stream-ordered).
+ * With periodic_boundary on a multilabel volume whose foreground is a single
+ * row, the wrap-around unions are skipped, as the binding's single-row
+ * shortcut (cc3d/fastcc3d.pyx:469-470, :644-679) ignores periodicity, unless
+ * CC3D_AMD_NO_SINGLE_ROW_RULE is set. */
+int cc3d_amd_label_begin(cc3d_amd_job** job, const void* in, int in_dtype, int64_t sx,
+                         int64_t sy, int64_t sz, int64_t connectivity, const void* delta,
+                         int flags, int periodic_boundary, int binary_image,
+                         void* workspace, size_t workspace_bytes, void* stream);
+/* Waits for stage 1 and returns its counters. */
+int cc3d_amd_job_stats(cc3d_amd_job* job, cc3d_amd_stats* stats);
+/* Stage 2: writes every voxel of `out` (out_dtype 1/2/3). */
+int cc3d_amd_label_finish(cc3d_amd_job* job, void* out, int out_dtype);
+/* Releases the job (and its workspace when the library allocated it). */
+void cc3d_amd_job_free(cc3d_amd_job* job);
+
+/* Union-find length the reference's first pass would allocate for these
+ * arguments (cc3d/cc3d.hpp:360-366 and twins, cc3d/cc3d_binary.hpp:47-51,
+ * :350-352, :527-530, :794-797, :964-967); 0 when that pass returns before
+ * labelling (max_labels == 0).  A provisional count >= this length is the
+ * reference's "maximum length exception" (cc3d/cc3d.hpp:107-111). */
+uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t connectivity,
+                                    int binary_image, int out_dtype);
+
+/* ---- follow-ons: statistics and dust ------------------------------------ */
+
+/* Max (unsigned view) or signed min/max of a label volume in HBM; used for
+ * N = np.max(labels) and the sign check of cc3d/fastcc3d.pyx:731-748. */
+int cc3d_amd_label_range(const void* labels, int dtype, int64_t voxels, int is_signed,
+                         uint64_t* umax, int64_t* smin, int64_t* smax, void* stream);
+
+/* Replaces cc3d.statistics (cc3d/fastcc3d.pyx:681-938).  Axes are the numpy
+ * axes (s0, s1, s2; s2 = 1 for ndim == 2); f_order selects which axis is
+ * fastest in memory.  Outputs (device): counts uint32[N+1];
+ * bbox uint32[(N+1)*2*ndim] as min,max per axis (min 0xFFFFFFFF, max 0 for
+ * absent labels); centroids float64[(N+1)*ndim] (NaN for absent labels);
+ * sums_scratch uint64[(N+1)*ndim]. */
+int cc3d_amd_statistics(const void* labels, int dtype, int64_t s0, int64_t s1, int64_t s2,
+                        int ndim, int f_order, uint64_t N, uint32_t* counts, uint32_t* bbox,
+                        double* centroids, unsigned long long* sums_scratch, void* stream);
+
+/* The masking step of cc3d.dust (cc3d/__init__.py:121-150): zero img where
+ * the voxel's component fails `size < lo` (ranged == 0) or
+ * `!(lo <= size < hi)` (ranged != 0), inverted when invert != 0.
+ * erase_scratch: uint8[N+1]; *n_masked (device uint64, pre-zeroed) receives
+ * the number of labels in 1..N that fail the test. */
+int cc3d_amd_dust_apply(void* img, int img_dtype, const void* labels, int label_dtype,
+                        int64_t voxels, const uint32_t* counts, uint64_t N, double lo, double hi,
+                        int ranged, int invert, uint8_t* erase_scratch,
+                        unsigned long long* n_masked, void* stream);
+const char* cc3d_amd_follow_last_error(void);
+
+const char* cc3d_amd_last_error(void);
+const char* cc3d_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CC3D_AMD_H */

@@ -1,0 +1,536 @@
+/*
+ * oracle.c -- CPU restatement of the cc3d labelling path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * bench's cpu_baseline leg may load this library, and only as the checker.
+ * The product path (the HIP library) never links, loads or calls it.
+ *
+ * What it restates (reference files cited per function below):
+ *   - cc3d/cc3d.hpp:287-315      estimate_provisional_label_count
+ *   - cc3d/cc3d.hpp:344-1421     multilabel first passes (26/18/6, 2D 4/8)
+ *   - cc3d/cc3d_binary.hpp       binary first passes incl. 2x2 block 2D-8
+ *   - cc3d/cc3d_continuous.hpp   continuous (|a-b| <= delta) first passes
+ *   - cc3d/cc3d.hpp:239-285      relabel (first-appearance numbering)
+ *
+ * The reference's result is determined by the set of union-find links its
+ * decision trees record (copies + unifies).  The restatement computes, per
+ * voxel, a link set whose transitive closure equals the reference's, then
+ * numbers components by the rank of their first voxel in memory order
+ * (x fastest): a component's first voxel always receives a fresh provisional
+ * label and relabel() assigns final labels in provisional order.
+ *
+ *   canonical  (multilabel, binary, continuous 3D / 2D-4):
+ *              p links to every backward neighbour q with
+ *                multilabel : v[q] == v[p]           (T's ==, v[p] != 0)
+ *                binary     : v[q] != 0
+ *                continuous : v[q] != 0 && match(v[p], v[q], delta)
+ *              plus, for continuous 26-connectivity, the exact-equality link
+ *              to the voxel below (cc3d/cc3d_continuous.hpp:147-150).
+ *   quirks     Q1 binary 2D-8 numbers by first 2x2 block
+ *                 (cc3d/cc3d_binary.hpp:955-1263)
+ *              Q2 binary 2D-4: row 0 links on value equality; later rows
+ *                 skip the left link when the up and up-left pixels are set
+ *                 (cc3d/cc3d_binary.hpp:822-935)
+ *              Q3 continuous 2D-8 links only the pixel above when it is equal
+ *                 or when both lie within delta of the global min/max
+ *                 (cc3d/cc3d_continuous.hpp:298-349)
+ *              Q6 binary 2D-8 leaves provisional block labels in background
+ *                 pixels outside the rewritten runs
+ *                 (cc3d/cc3d_binary.hpp:1233-1260)
+ * P, the provisional label count, is the number of voxels (blocks for Q1)
+ * whose link set is empty; the reference throws "maximum length exception"
+ * when P reaches its union-find length (cc3d/cc3d.hpp:107-111).
+ *
+ * Pinning: tests/test_oracle_*.py check every function here against the
+ * compiled reference (oracle/_ref), the reference's known-answer tests
+ * (tests/golden/), scipy.ndimage.label and brute-force BFS.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { DT_U8 = 0, DT_U16 = 1, DT_U32 = 2, DT_U64 = 3, DT_F32 = 4, DT_F64 = 5 };
+enum { M_MULTI = 0, M_BINARY = 1, M_CONT = 2 };
+
+/* ---------------- union-find (plain, sequential) ---------------- */
+static uint32_t uf_find(uint32_t* P, uint32_t a) {
+  while (P[a] != a) {
+    P[a] = P[P[a]];
+    a = P[a];
+  }
+  return a;
+}
+static void uf_union(uint32_t* P, uint32_t a, uint32_t b) {
+  a = uf_find(P, a);
+  b = uf_find(P, b);
+  if (a == b) return;
+  if (a < b) P[b] = a; else P[a] = b;
+}
+
+/* ---------------- typed value helpers ---------------- */
+typedef struct {
+  const void* p;
+  int dt;
+} vol_t;
+
+static inline int is_fg(const vol_t* v, int64_t i) {
+  switch (v->dt) {
+    case DT_U8: return ((const uint8_t*)v->p)[i] != 0;
+    case DT_U16: return ((const uint16_t*)v->p)[i] != 0;
+    case DT_U32: return ((const uint32_t*)v->p)[i] != 0;
+    case DT_U64: return ((const uint64_t*)v->p)[i] != 0;
+    case DT_F32: return ((const float*)v->p)[i] != 0.0f;
+    default: return ((const double*)v->p)[i] != 0.0;
+  }
+}
+static inline int is_eq(const vol_t* v, int64_t i, int64_t j) {
+  switch (v->dt) {
+    case DT_U8: return ((const uint8_t*)v->p)[i] == ((const uint8_t*)v->p)[j];
+    case DT_U16: return ((const uint16_t*)v->p)[i] == ((const uint16_t*)v->p)[j];
+    case DT_U32: return ((const uint32_t*)v->p)[i] == ((const uint32_t*)v->p)[j];
+    case DT_U64: return ((const uint64_t*)v->p)[i] == ((const uint64_t*)v->p)[j];
+    case DT_F32: return ((const float*)v->p)[i] == ((const float*)v->p)[j];
+    default: return ((const double*)v->p)[i] == ((const double*)v->p)[j];
+  }
+}
+/* cc3d/cc3d_continuous.hpp:79-88: unsigned -> (a>b ? a-b : b-a) <= d;
+ * float/double -> fabs(a - b) <= d evaluated in T. */
+static inline int is_match(const vol_t* v, int64_t i, int64_t j, uint64_t du,
+                           double df) {
+  switch (v->dt) {
+    case DT_U8: {
+      uint8_t a = ((const uint8_t*)v->p)[i], b = ((const uint8_t*)v->p)[j];
+      uint8_t d = (uint8_t)(a > b ? a - b : b - a);
+      return d <= (uint8_t)du;
+    }
+    case DT_U16: {
+      uint16_t a = ((const uint16_t*)v->p)[i], b = ((const uint16_t*)v->p)[j];
+      uint16_t d = (uint16_t)(a > b ? a - b : b - a);
+      return d <= (uint16_t)du;
+    }
+    case DT_U32: {
+      uint32_t a = ((const uint32_t*)v->p)[i], b = ((const uint32_t*)v->p)[j];
+      return (uint32_t)(a > b ? a - b : b - a) <= (uint32_t)du;
+    }
+    case DT_U64: {
+      uint64_t a = ((const uint64_t*)v->p)[i], b = ((const uint64_t*)v->p)[j];
+      return (a > b ? a - b : b - a) <= du;
+    }
+    case DT_F32: {
+      volatile float a = ((const float*)v->p)[i], b = ((const float*)v->p)[j];
+      volatile float d = a - b;
+      return fabsf(d) <= (float)df;
+    }
+    default: {
+      volatile double a = ((const double*)v->p)[i], b = ((const double*)v->p)[j];
+      volatile double d = a - b;
+      return fabs(d) <= df;
+    }
+  }
+}
+
+/* ---------------- estimate_provisional_label_count ----------------
+ * cc3d/cc3d.hpp:287-315: per row, count the first voxel if nonzero plus every
+ * x where v[x] != v[x-1] and v[x] != 0; track first/last rows with a count. */
+uint64_t oracle_epl(const void* in, int dt, int64_t sx, int64_t voxels,
+                    int64_t* first_row, int64_t* last_row) {
+  vol_t v = {in, dt};
+  int64_t f = -1, l = -1;
+  uint64_t count = 0;
+  int64_t row = 0;
+  for (int64_t loc = 0; loc < voxels; loc += sx, row++) {
+    uint64_t rc = is_fg(&v, loc);
+    for (int64_t x = 1; x < sx; x++)
+      rc += (!is_eq(&v, loc + x, loc + x - 1) && is_fg(&v, loc + x));
+    count += rc;
+    if (rc) {
+      if (f == -1) f = row;
+      l = row;
+    }
+  }
+  *first_row = f;
+  *last_row = l;
+  return count;
+}
+
+/* ---------------- neighbourhoods ---------------- */
+typedef struct {
+  int dx, dy, dz;
+} off_t3;
+
+static int backward_offsets(int conn, off_t3* o) {
+  int n = 0;
+  if (conn == 4 || conn == 8) {
+    o[n++] = (off_t3){-1, 0, 0};
+    o[n++] = (off_t3){0, -1, 0};
+    if (conn == 8) {
+      o[n++] = (off_t3){-1, -1, 0};
+      o[n++] = (off_t3){1, -1, 0};
+    }
+    return n;
+  }
+  for (int dz = -1; dz <= 0; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;
+        int m = abs(dx) + abs(dy) + abs(dz);
+        if (conn == 6 && m > 1) continue;
+        if (conn == 18 && m > 2) continue;
+        o[n++] = (off_t3){dx, dy, dz};
+      }
+  return n;
+}
+
+/* ---------------- global min/max for Q3 ----------------
+ * cc3d/cc3d_continuous.hpp:298-303 folds std::max/std::min over the raw
+ * array starting from element 0; std::max(a,b) = (a < b) ? b : a. */
+static void q3_minmax(const vol_t* v, int64_t voxels, double* gmin_f,
+                      double* gmax_f, uint64_t* gmin_u, uint64_t* gmax_u) {
+  if (v->dt == DT_F32) {
+    const float* a = (const float*)v->p;
+    float mx = a[0], mn = a[0];
+    for (int64_t i = 1; i < voxels; i++) {
+      mx = (a[i] < mx) ? mx : a[i];
+      mn = (mn < a[i]) ? mn : a[i];
+    }
+    *gmin_f = mn;
+    *gmax_f = mx;
+  } else if (v->dt == DT_F64) {
+    const double* a = (const double*)v->p;
+    double mx = a[0], mn = a[0];
+    for (int64_t i = 1; i < voxels; i++) {
+      mx = (a[i] < mx) ? mx : a[i];
+      mn = (mn < a[i]) ? mn : a[i];
+    }
+    *gmin_f = mn;
+    *gmax_f = mx;
+  } else {
+    uint64_t mx = 0, mn = 0;
+    for (int64_t i = 0; i < voxels; i++) {
+      uint64_t x;
+      switch (v->dt) {
+        case DT_U8: x = ((const uint8_t*)v->p)[i]; break;
+        case DT_U16: x = ((const uint16_t*)v->p)[i]; break;
+        case DT_U32: x = ((const uint32_t*)v->p)[i]; break;
+        default: x = ((const uint64_t*)v->p)[i]; break;
+      }
+      if (i == 0) mx = mn = x;
+      if (x > mx) mx = x;
+      if (x < mn) mn = x;
+    }
+    *gmin_u = mn;
+    *gmax_u = mx;
+  }
+}
+
+/* Q3 shortcut predicate: min(cur,B) - gmin <= delta && gmax - max(cur,B) <= delta
+ * in T arithmetic (cc3d/cc3d_continuous.hpp:341-349). */
+static int q3_span(const vol_t* v, int64_t p, int64_t b, uint64_t du, double df,
+                   double gmin_f, double gmax_f, uint64_t gmin_u, uint64_t gmax_u) {
+  if (v->dt == DT_F32) {
+    float c = ((const float*)v->p)[p], q = ((const float*)v->p)[b];
+    float lo = (q < c) ? q : c;  /* std::min(cur, B) */
+    float hi = (c < q) ? q : c;  /* std::max(cur, B) */
+    volatile float d1 = lo - (float)gmin_f;
+    volatile float d2 = (float)gmax_f - hi;
+    return d1 <= (float)df && d2 <= (float)df;
+  } else if (v->dt == DT_F64) {
+    double c = ((const double*)v->p)[p], q = ((const double*)v->p)[b];
+    double lo = (q < c) ? q : c;
+    double hi = (c < q) ? q : c;
+    volatile double d1 = lo - gmin_f;
+    volatile double d2 = gmax_f - hi;
+    return d1 <= df && d2 <= df;
+  } else {
+    uint64_t c, q, mask;
+    switch (v->dt) {
+      case DT_U8: c = ((const uint8_t*)v->p)[p]; q = ((const uint8_t*)v->p)[b]; mask = 0xFFu; break;
+      case DT_U16: c = ((const uint16_t*)v->p)[p]; q = ((const uint16_t*)v->p)[b]; mask = 0xFFFFu; break;
+      case DT_U32: c = ((const uint32_t*)v->p)[p]; q = ((const uint32_t*)v->p)[b]; mask = 0xFFFFFFFFu; break;
+      default: c = ((const uint64_t*)v->p)[p]; q = ((const uint64_t*)v->p)[b]; mask = ~0ull; break;
+    }
+    uint64_t lo = q < c ? q : c, hi = c < q ? q : c;
+    /* arithmetic is in T after integer promotion; results are non-negative */
+    return ((lo - gmin_u) & mask) <= (du & mask) && ((gmax_u - hi) & mask) <= (du & mask);
+  }
+}
+
+/* ---------------- Q1 / Q6 helpers (binary 2D-8, 2x2 blocks) ---------------- */
+
+/* Source block chosen by the block decision tree of
+ * cc3d/cc3d_binary.hpp:1029-1161 for the block at pixel (x, y) (x, y even),
+ * or -1 when the block takes a new provisional label, or -2 when the block
+ * holds no foreground.  Only needed to reproduce provisional labels for Q6
+ * and to count provisional labels for the length check. */
+static int64_t q1_source(const vol_t* v, int64_t x, int64_t y, int64_t sx,
+                         int64_t sy) {
+  const int64_t osx = (sx + 1) >> 1;
+  const int64_t ob = (x >> 1) + osx * (y >> 1);
+#define PX(xx, yy) is_fg(v, (xx) + sx * (yy))
+  const int Y = PX(x, y);
+  const int Z = (x < sx - 1) && PX(x + 1, y);
+  const int X = (y < sy - 1) && PX(x, y + 1);
+  const int W = (x < sx - 1) && (y < sy - 1) && PX(x + 1, y + 1);
+  const int64_t up = ob - osx, left = ob - 1, upleft = ob - 1 - osx,
+                upright = ob + 1 - osx;
+  if (Y) {
+    if (y > 0 && PX(x, y - 1)) return up;                         /* D */
+    if (y > 0 && x < sx - 1 && PX(x + 1, y - 1)) return up;       /* E */
+    if (x > 0 && PX(x - 1, y)) return left;                       /* B */
+    if (x > 0 && y > 0 && PX(x - 1, y - 1)) return upleft;        /* C */
+    if (x > 0 && y < sy - 1 && PX(x - 1, y + 1)) return left;     /* A */
+    if (x < sx - 2 && y > 0 && Z && PX(x + 2, y - 1)) return upright; /* Z&F */
+    return -1;
+  }
+  if (Z) {
+    if (y > 0 && PX(x + 1, y - 1)) return up;                     /* E */
+    if (y > 0 && PX(x, y - 1)) return up;                         /* D */
+    if (x < sx - 2 && y > 0 && PX(x + 2, y - 1)) return upright;  /* F */
+    if (X) {
+      if (x > 0 && PX(x - 1, y)) return left;                     /* B */
+      if (x > 0 && PX(x - 1, y + 1)) return left;                 /* A */
+      return -1;
+    }
+    return -1;
+  }
+  if (X) {
+    if (x > 0 && PX(x - 1, y)) return left;
+    if (x > 0 && PX(x - 1, y + 1)) return left;
+    return -1;
+  }
+  if (W) return -1;
+  return -2;
+#undef PX
+}
+
+/* compute_foreground_index (cc3d/cc3d.hpp:317-342) for one row */
+static void row_run(const vol_t* v, int64_t row, int64_t sx, uint32_t* s,
+                    uint32_t* e) {
+  uint32_t a = 0, b = 0;
+  int64_t loc = row * sx;
+  for (int64_t x = 0; x < sx; x++)
+    if (is_fg(v, loc + x)) {
+      a = (uint32_t)x;
+      break;
+    }
+  for (int64_t x = sx - 1; x >= (int64_t)a; x--)
+    if (is_fg(v, loc + x)) {
+      b = (uint32_t)(x + 1);
+      break;
+    }
+  *s = a;
+  *e = b;
+}
+
+/* ---------------- main entry ---------------- */
+
+/*
+ * oracle_label: labels `in` (already viewed as u8/u16/u32/u64/f32/f64, x
+ * fastest, shape sx*sy*sz) the way cc3d::connected_components3d<T,U>
+ * (cc3d/cc3d_continuous.hpp:394-455) does.  Labels are written as uint64 to
+ * out[]; *N gets the component count, *P the provisional label count.
+ * mode: 0 multilabel (delta == 0), 1 binary, 2 continuous.
+ * Returns 0, or -1 on bad arguments / allocation failure.
+ */
+int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
+                 int connectivity, int mode, uint64_t delta_u, double delta_f,
+                 int periodic, uint64_t* out, uint64_t* N, uint64_t* P) {
+  const int64_t sxy = sx * sy, V = sxy * sz;
+  vol_t v = {in, dt};
+  *N = 0;
+  *P = 0;
+  if (V <= 0) return 0;
+  if (V >= 0xFFFFFFFFll) return -1;
+  if ((connectivity == 4 || connectivity == 8) && sz != 1) return -1;
+
+  const int q1 = (mode == M_BINARY && connectivity == 8);
+  const int q2 = (mode == M_BINARY && connectivity == 4);
+  const int q3 = (mode == M_CONT && connectivity == 8);
+
+  uint32_t* par = (uint32_t*)malloc(sizeof(uint32_t) * V);
+  if (!par) return -1;
+  for (int64_t i = 0; i < V; i++) par[i] = (uint32_t)i;
+
+  off_t3 offs[13];
+  const int noff = backward_offsets(connectivity, offs);
+
+  double gmin_f = 0, gmax_f = 0;
+  uint64_t gmin_u = 0, gmax_u = 0;
+  if (q3) q3_minmax(&v, V, &gmin_f, &gmax_f, &gmin_u, &gmax_u);
+
+  uint64_t prov = 0;
+  for (int64_t z = 0; z < sz; z++)
+    for (int64_t y = 0; y < sy; y++)
+      for (int64_t x = 0; x < sx; x++) {
+        const int64_t p = x + sx * (y + sy * z);
+        if (!is_fg(&v, p)) continue;
+        int nlinks = 0;
+        if (q2) {
+          /* cc3d/cc3d_binary.hpp:822-935 */
+          if (y == 0) {
+            if (x > 0 && is_eq(&v, p, p - 1)) { uf_union(par, p, p - 1); nlinks++; }
+          } else {
+            const int B = x > 0 && is_fg(&v, p - 1);
+            const int C = is_fg(&v, p - sx);
+            const int D = x > 0 && is_fg(&v, p - 1 - sx);
+            if (C) { uf_union(par, p, p - sx); nlinks++; }
+            if (B && !(C && D)) { uf_union(par, p, p - 1); nlinks++; }
+          }
+        } else if (q3) {
+          /* cc3d/cc3d_continuous.hpp:335-384 */
+          if (y > 0 && is_eq(&v, p, p - sx)) {
+            uf_union(par, p, p - sx);
+            nlinks++;
+          } else if (y > 0 && is_fg(&v, p - sx) &&
+                     q3_span(&v, p, p - sx, delta_u, delta_f, gmin_f, gmax_f, gmin_u, gmax_u)) {
+            uf_union(par, p, p - sx);
+            nlinks++;
+          } else {
+            const int64_t cand[4] = {y > 0 ? p - sx : -1, (x > 0 && y > 0) ? p - 1 - sx : -1,
+                                     (x < sx - 1 && y > 0) ? p + 1 - sx : -1, x > 0 ? p - 1 : -1};
+            for (int k = 0; k < 4; k++) {
+              if (cand[k] < 0) continue;
+              if (is_fg(&v, cand[k]) && is_match(&v, p, cand[k], delta_u, delta_f)) {
+                uf_union(par, p, cand[k]);
+                nlinks++;
+              }
+            }
+          }
+        } else {
+          for (int k = 0; k < noff; k++) {
+            const int64_t qx = x + offs[k].dx, qy = y + offs[k].dy, qz = z + offs[k].dz;
+            if (qx < 0 || qx >= sx || qy < 0 || qy >= sy || qz < 0) continue;
+            const int64_t q = qx + sx * (qy + sy * qz);
+            int lk;
+            if (mode == M_MULTI) lk = is_eq(&v, p, q);
+            else if (mode == M_BINARY) lk = is_fg(&v, q);
+            else {
+              lk = is_fg(&v, q) && is_match(&v, p, q, delta_u, delta_f);
+              /* cc3d/cc3d_continuous.hpp:147-150 */
+              if (connectivity == 26 && offs[k].dx == 0 && offs[k].dy == 0 && is_eq(&v, p, q))
+                lk = 1;
+            }
+            if (lk) {
+              uf_union(par, p, q);
+              nlinks++;
+            }
+          }
+        }
+        if (!q1 && nlinks == 0) prov++;
+      }
+
+  /* periodic boundary links: cc3d/cc3d.hpp:1048-1073, :1265-1277,
+   * :1377-1418; cc3d/cc3d_binary.hpp:733-758, :938-950, :1163-1210 */
+  if (periodic && mode != M_CONT) {
+#define PLINK(a, b)                                                         \
+  do {                                                                      \
+    int64_t _a = (a), _b = (b);                                             \
+    if (is_fg(&v, _a) && (mode == M_BINARY ? is_fg(&v, _b) : is_eq(&v, _a, _b))) \
+      uf_union(par, _a, _b);                                                \
+  } while (0)
+    if (connectivity == 6) {
+      for (int64_t z = 0; z < sz; z++)
+        for (int64_t y = 0; y < sy; y++) PLINK(sx * (y + sy * z), sx * (y + sy * z) + sx - 1);
+      for (int64_t z = 0; z < sz; z++)
+        for (int64_t x = 0; x < sx; x++) PLINK(x + sxy * z, x + sxy * z + sx * (sy - 1));
+      for (int64_t y = 0; y < sy; y++)
+        for (int64_t x = 0; x < sx; x++) PLINK(x + sx * y, x + sx * y + sxy * (sz - 1));
+    } else if (connectivity == 4) {
+      for (int64_t x = 0; x < sx; x++) PLINK(x, x + sx * (sy - 1));
+      for (int64_t y = 0; y < sy; y++) PLINK(sx * y, sx * y + sx - 1);
+    } else if (connectivity == 8) {
+      for (int64_t x = 0; x < sx; x++) {
+        if (x > 0) PLINK(x, x - 1 + sx * (sy - 1));
+        PLINK(x, x + sx * (sy - 1));
+        if (x < sx - 1) PLINK(x, x + 1 + sx * (sy - 1));
+      }
+      PLINK(0, V - 1);
+      PLINK(sx - 1, sx * (sy - 1));
+      for (int64_t y = 0; y < sy; y++) {
+        PLINK(sx * y, sx * y + sx - 1);
+        if (y > 0) PLINK(sx * y, sx * y + sx - 1 - sx);
+        if (y < sy - 1) PLINK(sx * y, sx * y + sx - 1 + sx);
+      }
+    }
+#undef PLINK
+  }
+
+  /* numbering key: voxel index, or for Q1 the 2x2 block key */
+  const int64_t osx = (sx + 1) >> 1, osy = (sy + 1) >> 1;
+  const int64_t K = q1 ? 4 * osx * osy : V;
+  uint32_t* minkey = (uint32_t*)malloc(sizeof(uint32_t) * V);
+  uint8_t* used = (uint8_t*)calloc(K, 1);
+  uint64_t* rank = (uint64_t*)malloc(sizeof(uint64_t) * K);
+  if (!minkey || !used || !rank) {
+    free(par); free(minkey); free(used); free(rank);
+    return -1;
+  }
+  for (int64_t i = 0; i < V; i++) minkey[i] = 0xFFFFFFFFu;
+  for (int64_t p = 0; p < V; p++) {
+    if (!is_fg(&v, p)) continue;
+    uint32_t key;
+    if (q1) {
+      const int64_t x = p % sx, y = p / sx;
+      key = (uint32_t)((((y >> 1) * osx + (x >> 1)) << 2) | ((y & 1) << 1) | (x & 1));
+    } else {
+      key = (uint32_t)p;
+    }
+    uint32_t r = uf_find(par, (uint32_t)p);
+    if (key < minkey[r]) minkey[r] = key;
+  }
+  for (int64_t p = 0; p < V; p++)
+    if (is_fg(&v, p) && par[p] == (uint32_t)p) used[minkey[p]] = 1;
+  uint64_t n = 0;
+  for (int64_t k = 0; k < K; k++) {
+    if (used[k]) n++;
+    rank[k] = n;
+  }
+  for (int64_t p = 0; p < V; p++)
+    out[p] = is_fg(&v, p) ? rank[minkey[uf_find(par, (uint32_t)p)]] : 0;
+  *N = n;
+
+  if (q1) {
+    /* provisional block labels in block raster order (for P and Q6) */
+    const int64_t nb = osx * osy;
+    uint64_t* bl = (uint64_t*)calloc(nb, sizeof(uint64_t));
+    if (!bl) {
+      free(par); free(minkey); free(used); free(rank);
+      return -1;
+    }
+    for (int64_t y = 0; y < sy; y += 2)
+      for (int64_t x = 0; x < sx; x += 2) {
+        const int64_t ob = (x >> 1) + osx * (y >> 1);
+        const int64_t src = q1_source(&v, x, y, sx, sy);
+        if (src == -2) continue;
+        if (src == -1) bl[ob] = ++prov;
+        else bl[ob] = bl[src];
+      }
+    /* Q6: cc3d/cc3d_binary.hpp:1233-1260 rewrites rows with y*sx < V/4
+     * entirely, later rows only on [xstart, xend) of rows y and y+1; the
+     * first osx*osy entries of out held the block labels. */
+    const int64_t target = V >> 2;
+    for (int64_t y = 0; y < sy; y++) {
+      if (y * sx < target) continue;
+      const int64_t ymax = (y < sy - 1) ? y + 1 : y;
+      uint32_t s0, e0, s1, e1;
+      row_run(&v, y, sx, &s0, &e0);
+      row_run(&v, ymax, sx, &s1, &e1);
+      const int64_t xs = (int64_t)(s0 < s1 ? s0 : s1) & ~(int64_t)1;
+      const int64_t xe = (int64_t)(e0 > e1 ? e0 : e1);
+      for (int64_t x = 0; x < sx; x++) {
+        if (x >= xs && x < xe) continue;
+        const int64_t loc = x + sx * y;
+        if (loc < nb) out[loc] = bl[loc];
+      }
+    }
+    free(bl);
+  }
+  *P = prov;
+
+  free(par);
+  free(minkey);
+  free(used);
+  free(rank);
+  return 0;
+}

@@ -1,0 +1,341 @@
+"""Checker-side bindings: the C restatement (oracle.c) and the compiled reference.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+bench's ``cpu_baseline`` leg import this module, and only as the checker.  The
+product package never imports it.
+
+Two backends share one Python restatement of the binding layer
+(cc3d/fastcc3d.pyx:244-642):
+
+* ``"oracle"`` -- oracle/build/liboracle.so, the C restatement of the
+  labelling core (cc3d/cc3d.hpp, cc3d/cc3d_binary.hpp,
+  cc3d/cc3d_continuous.hpp).
+* ``"ref"``    -- oracle/_ref/libcc3d_ref.so, the unmodified reference headers
+  compiled in place (oracle/Makefile).  Present wherever the reference tree
+  was available at build time; the library itself travels with the repo.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcc3d_ref.so")
+
+DT_CODE = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
+           np.dtype(np.uint64): 3, np.dtype(np.float32): 4, np.dtype(np.float64): 5}
+MODE_MULTI, MODE_BINARY, MODE_CONT = 0, 1, 2
+
+_oracle = None
+_ref = None
+
+
+class DimensionError(Exception):
+  pass
+
+
+def build(force: bool = False) -> None:
+  """Compile the oracle (and, where the reference tree exists, oracle/_ref)."""
+  import subprocess
+  subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def _lib_oracle():
+  global _oracle
+  if _oracle is None:
+    if not os.path.exists(ORACLE_SO):
+      build()
+    lib = ctypes.CDLL(ORACLE_SO)
+    lib.oracle_label.restype = ctypes.c_int
+    lib.oracle_label.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+      ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
+      ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+    ]
+    lib.oracle_epl.restype = ctypes.c_uint64
+    lib.oracle_epl.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+    ]
+    _oracle = lib
+  return _oracle
+
+
+def have_ref() -> bool:
+  return os.path.exists(REF_SO)
+
+
+def _lib_ref():
+  global _ref
+  if _ref is None:
+    lib = ctypes.CDLL(REF_SO)
+    lib.ref_connected_components3d.restype = ctypes.c_int
+    lib.ref_connected_components3d.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+      ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64,
+      ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+      ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+    ]
+    lib.ref_estimate_provisional_label_count.restype = ctypes.c_uint64
+    lib.ref_estimate_provisional_label_count.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+    ]
+    _ref = lib
+  return _ref
+
+
+# ---------------------------------------------------------------------------
+# core calls (input already viewed as u8/u16/u32/u64/f32/f64, x fastest)
+# ---------------------------------------------------------------------------
+
+def core_label(flat: np.ndarray, sx: int, sy: int, sz: int, connectivity: int,
+               mode: int, delta=0, periodic: bool = False) -> Tuple[np.ndarray, int, int]:
+  """oracle.c: labels as uint64, N, provisional count P."""
+  lib = _lib_oracle()
+  flat = np.ascontiguousarray(flat).reshape(-1)
+  out = np.zeros(flat.size, dtype=np.uint64)
+  N = ctypes.c_uint64(0)
+  P = ctypes.c_uint64(0)
+  du, df = _delta_pair(flat.dtype, delta)
+  rc = lib.oracle_label(flat.ctypes.data, DT_CODE[flat.dtype], sx, sy, sz, connectivity,
+                        mode, du, df, int(bool(periodic)), out.ctypes.data,
+                        ctypes.byref(N), ctypes.byref(P))
+  if rc != 0:
+    raise RuntimeError("oracle_label failed")
+  return out, int(N.value), int(P.value)
+
+
+def core_epl(flat: np.ndarray, sx: int):
+  lib = _lib_oracle()
+  flat = np.ascontiguousarray(flat).reshape(-1)
+  f = ctypes.c_int64(0)
+  l = ctypes.c_int64(0)
+  e = lib.oracle_epl(flat.ctypes.data, DT_CODE[flat.dtype], sx, flat.size,
+                     ctypes.byref(f), ctypes.byref(l))
+  return int(e), int(f.value), int(l.value)
+
+
+def ref_core_label(flat: np.ndarray, sx: int, sy: int, sz: int, max_labels: int,
+                   connectivity: int, delta, out_dtype, periodic: bool, binary: bool):
+  """The compiled reference: (labels in out_dtype, N) or raises RuntimeError."""
+  lib = _lib_ref()
+  flat = np.ascontiguousarray(flat).reshape(-1)
+  out = np.zeros(flat.size, dtype=out_dtype)
+  N = ctypes.c_uint64(0)
+  du, df = _delta_pair(flat.dtype, delta)
+  err = ctypes.create_string_buffer(256)
+  rc = lib.ref_connected_components3d(
+    flat.ctypes.data, DT_CODE[flat.dtype], sx, sy, sz, max_labels, connectivity,
+    df, du, out.ctypes.data, DT_CODE[np.dtype(out_dtype)], ctypes.byref(N),
+    int(bool(periodic)), int(bool(binary)), err, 256)
+  if rc != 0:
+    raise RuntimeError(err.value.decode())
+  return out, int(N.value)
+
+
+def ref_core_epl(flat: np.ndarray, sx: int):
+  lib = _lib_ref()
+  flat = np.ascontiguousarray(flat).reshape(-1)
+  f = ctypes.c_int64(0)
+  l = ctypes.c_int64(0)
+  e = lib.ref_estimate_provisional_label_count(
+    flat.ctypes.data, DT_CODE[flat.dtype], sx, flat.size, ctypes.byref(f), ctypes.byref(l))
+  return int(e), int(f.value), int(l.value)
+
+
+def _delta_pair(dtype, delta):
+  if np.dtype(dtype).kind == "f":
+    return 0, float(delta)
+  d = int(delta)
+  if d < 0:
+    raise OverflowError("can't convert negative value to unsigned delta")
+  return d & 0xFFFFFFFFFFFFFFFF, float(d)
+
+
+# ---------------------------------------------------------------------------
+# union-find length of each reference first pass (for the P >= L check)
+# ---------------------------------------------------------------------------
+
+def uf_length(max_labels: int, voxels: int, connectivity: int, binary: bool,
+              out_dtype) -> Optional[int]:
+  """DisjointSet length the reference allocates, or None when it returns early.
+
+  cc3d/cc3d.hpp:360-366 (and the 18/6/2D twins), cc3d/cc3d_binary.hpp:47-51,
+  :350-352, :527-530, :794-797, :964-967, cc3d/cc3d_continuous.hpp:105-107.
+  """
+  omax = int(np.iinfo(out_dtype).max)
+  if binary and connectivity == 8:
+    ml = max(min(max_labels + 1, (voxels >> 1) + 1), 1) + 1
+    return min(ml, omax)
+  if max_labels == 0:
+    return None
+  ml = max_labels + 1
+  if binary and connectivity == 26:
+    ml = min(ml, (voxels >> 2) + 1) + 1
+  elif binary and connectivity in (6, 4):
+    ml = min(ml, (voxels >> 1) + 1) + 1
+  else:
+    ml = min(ml, voxels + 1)
+    if connectivity == 8:
+      ml = max(ml, 1)
+  return min(ml, omax)
+
+
+# ---------------------------------------------------------------------------
+# restatement of the binding layer, cc3d/fastcc3d.pyx:244-642
+# ---------------------------------------------------------------------------
+
+def _even_ceil(n: int) -> int:
+  return n << 1 if n & 1 else n
+
+
+def _view_unsigned(data: np.ndarray) -> np.ndarray:
+  dt = data.dtype
+  if dt in (np.uint64, np.int64):
+    return data.view(np.uint64)
+  if dt in (np.uint32, np.int32):
+    return data.view(np.uint32)
+  if dt in (np.uint16, np.int16):
+    return data.view(np.uint16)
+  if dt in (np.uint8, np.int8, np.bool_):
+    return data.view(np.uint8)
+  if dt in (np.float32, np.float64):
+    return data
+  raise TypeError(f"Type {dt} is not currently supported.")
+
+
+def estimate_provisional_labels(data: np.ndarray, backend: str = "oracle"):
+  """cc3d/fastcc3d.pyx:168-242."""
+  sx = data.shape[0] if data.flags.f_contiguous else data.shape[-1]
+  flat = np.ravel(_view_unsigned(np.asarray(data)), order="K")
+  if backend == "ref":
+    return ref_core_epl(flat, sx)
+  return core_epl(flat, sx)
+
+
+def connected_components(data, max_labels=-1, connectivity=26, return_N=False,
+                         delta=0, out_dtype=None, periodic_boundary=False,
+                         binary_image=False, backend: str = "oracle"):
+  """Checker-side cc3d.connected_components (cc3d/fastcc3d.pyx:244-626)."""
+  data = np.asarray(data)
+  dims = data.ndim
+  if dims not in (1, 2, 3):
+    raise DimensionError(f"Only 1D, 2D, and 3D arrays supported. Got: {dims}")
+  if dims == 2 and connectivity not in (4, 8, 6, 18, 26):
+    raise ValueError(f"Only 4, 8, and 6, 18, 26 connectivities are supported for 2D images. Got: {connectivity}")
+  elif dims != 2 and connectivity not in (6, 18, 26):
+    raise ValueError(f"Only 6, 18, and 26 connectivities are supported for 3D images. Got: {connectivity}")
+  if periodic_boundary and connectivity not in (4, 8, 6):
+    raise ValueError(f"periodic_boundary is not yet implemented for {connectivity}-connectivity.")
+  if periodic_boundary and delta != 0:
+    raise ValueError("periodic_boundary is not yet implemented continuous data.")
+  if data.size == 0:
+    dt = out_dtype if out_dtype is not None else data.dtype
+    out = np.zeros((0,), dtype=dt)
+    return (out, 0) if return_N else out
+
+  order = "F" if data.flags.f_contiguous else "C"
+  while data.ndim < 3:
+    data = data[np.newaxis, ...] if order == "C" else data[..., np.newaxis]
+  if not data.flags.c_contiguous and not data.flags.f_contiguous:
+    data = np.copy(data, order=order)
+  if data.dtype == np.float16:
+    if delta == 0:
+      data = data.view(np.uint16)
+    else:
+      raise TypeError("float16 is not supported for continuous images (delta != 0).")
+
+  shape = list(data.shape)
+  if order == "C":
+    shape.reverse()
+  sx, sy, sz = shape
+  voxels = sx * sy * sz
+  dtype = data.dtype
+  binary_image = bool(binary_image) or dtype == np.bool_
+
+  if binary_image:
+    epl, ffr, lfr = voxels, 0, sy
+  else:
+    epl, ffr, lfr = estimate_provisional_labels(data, backend)
+  max_labels = min(epl, voxels)
+
+  if np.issubdtype(dtype, np.floating):
+    delta = float(delta)
+    binary_image = binary_image or (delta == np.finfo(dtype).max)
+  else:
+    delta = int(delta)
+    binary_image = binary_image or (delta == np.iinfo(dtype).max)
+
+  ufv = _even_ceil(data.shape[0]) * _even_ceil(data.shape[1]) * _even_ceil(data.shape[2])
+  if binary_image:
+    if connectivity in (4, 6):
+      max_labels = min(max_labels, (ufv // 2) + 1)
+    else:  # the reference's `connectivity == (8,18)` is never true
+      max_labels = min(max_labels, (ufv // 8) + 1)
+
+  if out_dtype is not None:
+    out_dtype = np.dtype(out_dtype)
+    if out_dtype not in (np.uint16, np.uint32, np.uint64):
+      raise ValueError(f"Explicitly defined out_dtype ({out_dtype}) must be one of: np.uint16, np.uint32, np.uint64")
+    if np.iinfo(out_dtype).max < max_labels:
+      raise ValueError(f"Explicitly defined out_dtype ({out_dtype}) is too small to contain the estimated maximum number of labels ({max_labels}).")
+  elif max_labels < np.iinfo(np.uint16).max:
+    out_dtype = np.dtype(np.uint16)
+  elif max_labels < np.iinfo(np.uint32).max:
+    out_dtype = np.dtype(np.uint32)
+  else:
+    out_dtype = np.dtype(np.uint64)
+
+  flat = np.ravel(_view_unsigned(data), order="K")
+  mode = MODE_BINARY if binary_image else (MODE_MULTI if delta == 0 else MODE_CONT)
+
+  if delta == 0 and ffr == lfr and ffr >= 0:
+    out, N = _special_row(ffr, sx, sy, flat, out_dtype)
+  elif backend == "ref":
+    out, N = ref_core_label(flat, sx, sy, sz, max_labels, connectivity, delta,
+                            out_dtype, periodic_boundary, binary_image)
+  else:
+    L = uf_length(max_labels, voxels, connectivity, binary_image, out_dtype)
+    if L is None:
+      out, N = np.zeros(voxels, dtype=out_dtype), 0
+    else:
+      if connectivity in (4, 8) and sz != 1:
+        raise RuntimeError("sz must be 1 for 2D connectivities.")
+      lab, N, P = core_label(flat, sx, sy, sz, connectivity, mode, delta, periodic_boundary)
+      if P >= L:
+        raise RuntimeError("maximum length exception")
+      out = lab.astype(out_dtype)
+
+  if dims == 3:
+    out = out.reshape((sz, sy, sx) if order == "C" else (sx, sy, sz), order=order)
+  elif dims == 2:
+    out = out.reshape((sy, sx) if order == "C" else (sx, sy), order=order)
+  else:
+    out = out.reshape((sx,), order=order)
+  return (out, N) if return_N else out
+
+
+def _special_row(row: int, sx: int, sy: int, flat: np.ndarray, out_dtype):
+  """cc3d/fastcc3d.pyx:644-679 on integer rows: runs of equal nonzero values."""
+  out = np.zeros(flat.size, dtype=out_dtype)
+  start = row * sx
+  N = 0
+  last = 0
+  for i in range(sx):
+    val = flat[start + i]
+    if val == 0:
+      last = 0
+      continue
+    elif val == last:
+      out[start + i] = N
+      continue
+    else:
+      N += 1
+      out[start + i] = N
+      last = val
+  return out, N

@@ -1,0 +1,115 @@
+// C-ABI shim over the reference cc3d headers, built only into oracle/_ref/.
+//
+// Test infrastructure: this file is the checker's binding to the unmodified
+// reference implementation (cc3d/cc3d_continuous.hpp:394-455 and
+// cc3d/cc3d.hpp:287-315), compiled in place from the reference tree by
+// oracle/Makefile.  Nothing in the product links or loads it; tests, the
+// smoke check and the bench's CPU-baseline leg may.
+//
+// Mirrors the Cython dispatch in cc3d/fastcc3d.pyx:471-613: the input is
+// already viewed as one of u8/u16/u32/u64/f32/f64 and the output is one of
+// u16/u32/u64.  Exceptions (std::runtime_error, e.g. the union-find
+// "maximum length exception" of cc3d/cc3d.hpp:107-111) come back as status 1
+// with the message copied into errbuf.
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+
+#include "cc3d_continuous.hpp"
+
+namespace {
+
+enum : int { DT_U8 = 0, DT_U16 = 1, DT_U32 = 2, DT_U64 = 3, DT_F32 = 4, DT_F64 = 5 };
+
+template <typename T, typename U>
+void run(const void* in, int64_t sx, int64_t sy, int64_t sz, int64_t max_labels,
+         int64_t connectivity, T delta, void* out, uint64_t* N, bool periodic,
+         bool binary) {
+  size_t n = 0;
+  cc3d::connected_components3d<T, U>(
+      const_cast<T*>(static_cast<const T*>(in)), sx, sy, sz, max_labels,
+      connectivity, delta, static_cast<U*>(out), n, periodic, binary);
+  *N = n;
+}
+
+template <typename T>
+void run_out(const void* in, int64_t sx, int64_t sy, int64_t sz,
+             int64_t max_labels, int64_t connectivity, T delta, void* out,
+             int out_dtype, uint64_t* N, bool periodic, bool binary) {
+  if (out_dtype == DT_U16)
+    run<T, uint16_t>(in, sx, sy, sz, max_labels, connectivity, delta, out, N, periodic, binary);
+  else if (out_dtype == DT_U32)
+    run<T, uint32_t>(in, sx, sy, sz, max_labels, connectivity, delta, out, N, periodic, binary);
+  else if (out_dtype == DT_U64)
+    run<T, uint64_t>(in, sx, sy, sz, max_labels, connectivity, delta, out, N, periodic, binary);
+  else
+    throw std::invalid_argument("bad out dtype");
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0 on success, 1 on a C++ exception (message in errbuf).
+int ref_connected_components3d(const void* in, int in_dtype, int64_t sx,
+                               int64_t sy, int64_t sz, int64_t max_labels,
+                               int64_t connectivity, double delta_f,
+                               uint64_t delta_u, void* out, int out_dtype,
+                               uint64_t* N, int periodic, int binary,
+                               char* errbuf, int errlen) {
+  try {
+    switch (in_dtype) {
+      case DT_U8:
+        run_out<uint8_t>(in, sx, sy, sz, max_labels, connectivity, (uint8_t)delta_u, out, out_dtype, N, periodic, binary);
+        break;
+      case DT_U16:
+        run_out<uint16_t>(in, sx, sy, sz, max_labels, connectivity, (uint16_t)delta_u, out, out_dtype, N, periodic, binary);
+        break;
+      case DT_U32:
+        run_out<uint32_t>(in, sx, sy, sz, max_labels, connectivity, (uint32_t)delta_u, out, out_dtype, N, periodic, binary);
+        break;
+      case DT_U64:
+        run_out<uint64_t>(in, sx, sy, sz, max_labels, connectivity, (uint64_t)delta_u, out, out_dtype, N, periodic, binary);
+        break;
+      case DT_F32:
+        run_out<float>(in, sx, sy, sz, max_labels, connectivity, (float)delta_f, out, out_dtype, N, periodic, binary);
+        break;
+      case DT_F64:
+        run_out<double>(in, sx, sy, sz, max_labels, connectivity, delta_f, out, out_dtype, N, periodic, binary);
+        break;
+      default:
+        throw std::invalid_argument("bad input dtype");
+    }
+  } catch (const std::exception& e) {
+    if (errbuf && errlen > 0) {
+      std::strncpy(errbuf, e.what(), errlen - 1);
+      errbuf[errlen - 1] = 0;
+    }
+    return 1;
+  }
+  return 0;
+}
+
+// cc3d/cc3d.hpp:287-315 through cc3d/fastcc3d.pyx:168-242.
+uint64_t ref_estimate_provisional_label_count(const void* in, int in_dtype,
+                                              int64_t sx, int64_t voxels,
+                                              int64_t* first_row,
+                                              int64_t* last_row) {
+  int64_t f = 0, l = 0;
+  size_t r = 0;
+  switch (in_dtype) {
+    case DT_U8: r = cc3d::estimate_provisional_label_count<uint8_t>((uint8_t*)in, sx, voxels, f, l); break;
+    case DT_U16: r = cc3d::estimate_provisional_label_count<uint16_t>((uint16_t*)in, sx, voxels, f, l); break;
+    case DT_U32: r = cc3d::estimate_provisional_label_count<uint32_t>((uint32_t*)in, sx, voxels, f, l); break;
+    case DT_U64: r = cc3d::estimate_provisional_label_count<uint64_t>((uint64_t*)in, sx, voxels, f, l); break;
+    case DT_F32: r = cc3d::estimate_provisional_label_count<float>((float*)in, sx, voxels, f, l); break;
+    case DT_F64: r = cc3d::estimate_provisional_label_count<double>((double*)in, sx, voxels, f, l); break;
+    default: return 0;
+  }
+  *first_row = f;
+  *last_row = l;
+  return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+"""GPU parity: cc3d (HIP library) against the oracle and the compiled reference.
+
+Every case runs the product through its public Python surface (which calls the
+C ABI of libcc3d_amd.so) and the checker through oracle.connected_components
+(the same binding-layer restatement over oracle.c, or over the compiled
+reference in oracle/_ref when that library is present).  The bar is bit-exact:
+same labels, same N, same output dtype, same exception type.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(fn, arr, kw):
+  try:
+    out = fn(arr, return_N=True, **kw)
+    return ("ok", out[0], out[1])
+  except Exception as e:  # error parity is part of the contract
+    return ("err", type(e).__name__, str(e))
+
+
+def _same(a, b):
+  if a[0] != b[0]:
+    return False
+  if a[0] == "err":
+    return a[1] == b[1]
+  return a[2] == b[2] and a[1].dtype == b[1].dtype and a[1].shape == b[1].shape and np.array_equal(a[1], b[1])
+
+
+def _check(cc3d, arr, kw, tag=""):
+  print(f"case {tag} shape={arr.shape} dtype={arr.dtype} F={arr.flags.f_contiguous} kw={kw}", flush=True)
+  got = _run(cc3d.connected_components, arr, kw)
+  exp = _run(lambda a, **k: oracle.connected_components(a, backend="oracle", **k), arr, kw)
+  assert _same(got, exp), f"{tag}: product {got[0]} vs oracle {exp[0]}: {got[2] if got[0]=='ok' else got} / {exp[2] if exp[0]=='ok' else exp}"
+  if oracle.have_ref():
+    ref = _run(lambda a, **k: oracle.connected_components(a, backend="ref", **k), arr, kw)
+    assert _same(got, ref), f"{tag}: product vs reference build differ"
+
+
+def _rand_case(rng, big=False):
+  nd = int(rng.choice([2, 3]))
+  if nd == 2:
+    hi = 200 if big else 70
+    shape = tuple(int(s) for s in rng.integers(1, hi, size=2))
+    conn = int(rng.choice([4, 8, 6, 18, 26]))
+  else:
+    hi = 60 if big else 24
+    shape = tuple(int(s) for s in rng.integers(1, hi, size=3))
+    conn = int(rng.choice([6, 18, 26]))
+  dt = rng.choice([np.uint8, np.uint16, np.int32, np.uint32, np.uint64, np.int64, np.float32, np.float64,
+                   np.int8, np.int16, bool, np.float16])
+  kind = str(rng.choice(["multi", "binary", "cont"]))
+  nvals = int(rng.integers(2, 6))
+  arr = rng.integers(0, nvals, size=shape)
+  arr = (arr > 0) if dt == bool else arr.astype(dt)
+  if rng.random() < 0.5:
+    arr = np.asfortranarray(arr)
+  kw = dict(connectivity=conn)
+  if kind == "binary":
+    kw["binary_image"] = True
+  if kind == "cont" and dt not in (bool, np.float16):
+    kw["delta"] = int(rng.integers(1, 3))
+  if kind != "cont" and conn in (4, 6, 8) and rng.random() < 0.3:
+    kw["periodic_boundary"] = True
+  if rng.random() < 0.3:
+    kw["out_dtype"] = [np.uint16, np.uint32, np.uint64][int(rng.integers(0, 3))]
+  return arr, kw
+
+
+def test_random_small(gpu):
+  import cc3d
+  rng = np.random.default_rng(12345)
+  for i in range(400):
+    arr, kw = _rand_case(rng)
+    _check(cc3d, arr, kw, f"small#{i}")
+
+
+def test_random_tile_crossing(gpu):
+  """Shapes that span several 32x8x8 (3D) / 64x32 (2D) tiles with partial edges."""
+  import cc3d
+  rng = np.random.default_rng(777)
+  for i in range(80):
+    arr, kw = _rand_case(rng, big=True)
+    _check(cc3d, arr, kw, f"big#{i}")
+
+
+@pytest.mark.parametrize("conn", [26, 18, 6])
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_voronoi_multilabel(gpu, conn, order):
+  import cc3d
+  import synth
+  vol = synth.voronoi((97, 83, 71), 400, seed=5)
+  if order == "C":
+    vol = np.ascontiguousarray(vol)
+  _check(cc3d, vol, dict(connectivity=conn, out_dtype=np.uint32), "voronoi")
+
+
+@pytest.mark.parametrize("p", [0.1, 0.3, 0.5])
+@pytest.mark.parametrize("conn", [6, 26])
+def test_bernoulli_binary(gpu, p, conn):
+  import cc3d
+  import synth
+  vol = synth.bernoulli((130, 70, 50), p, seed=int(p * 10) + conn)
+  _check(cc3d, vol, dict(connectivity=conn, binary_image=True, out_dtype=np.uint32), "bernoulli")
+
+
+def test_bernoulli_2d8(gpu):
+  import cc3d
+  import synth
+  for shape in [(512, 512), (333, 517), (1, 129), (129, 1)]:
+    vol = synth.bernoulli(shape, 0.5, seed=9)
+    _check(cc3d, vol, dict(connectivity=8, binary_image=True, out_dtype=np.uint32), "bern2d8")
+
+
+def test_tones_continuous(gpu):
+  import cc3d
+  import synth
+  vol = synth.tones((90, 70, 40), 150, seed=4)
+  _check(cc3d, vol, dict(connectivity=26, delta=20, out_dtype=np.uint32), "tones")
+
+
+def test_config0_voronoi_256(gpu):
+  """256^3 uint32 Voronoi, 1500 seeds, membranes 0.8, salt 2 %, 26-conn, uint32, return_N."""
+  import cc3d
+  import synth
+  vol = synth.voronoi((256, 256, 256), 1500, seed=2024)
+  _check(cc3d, vol, dict(connectivity=26, out_dtype=np.uint32), "config0")
+
+
+def test_device_tensor_in_out(gpu):
+  import torch
+  import cc3d
+  import synth
+  vol = synth.voronoi((64, 48, 40), 100, seed=8)
+  exp, expN = oracle.connected_components(vol, connectivity=18, return_N=True)
+  flat = torch.from_numpy(np.ravel(vol, order="F")).cuda()
+  t = flat.as_strided(vol.shape, (1, vol.shape[0], vol.shape[0] * vol.shape[1]))
+  out, N = cc3d.connected_components(t, connectivity=18, return_N=True)
+  assert out.device.type == "cuda"
+  assert N == expN
+  assert np.array_equal(out.cpu().numpy(), exp)
+  tc = torch.from_numpy(np.ascontiguousarray(vol)).cuda()
+  out2 = cc3d.connected_components(tc, connectivity=18)
+  exp2 = oracle.connected_components(np.ascontiguousarray(vol), connectivity=18)
+  assert np.array_equal(out2.cpu().numpy(), exp2)
