@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 512^3 uint32 26-connected multilabel labelling on one MI355X.
+
+Workload (the labelling path's headline configuration): a 512x512x512 uint32
+Voronoi volume with 12,000 uniform seeds (the seed density of the 256^3,
+1,500-seed case), seed IDs uniform in [1, 2^31), membranes (a voxel with a
+different-region 26-neighbour is zeroed with probability 0.8) and 2 % salt
+(voxels overwritten by the ID of a random seed); x fastest; 26-connectivity,
+out_dtype uint32, return_N.  Generated deterministically (synth/synth.c).
+
+One step = one cc3d.connected_components call on the volume already resident
+in HBM (a device tensor), returning a device uint32 label volume and N.
+
+Prints one JSON line.  Also reports:
+  roofline      the dominant kernel's algorithmic bytes / its mean duration,
+                measured with HIP events on the launch stream during the timed
+                steps; traffic from the committed rocprofv3 counter summary
+  path_roofline the whole call: 8 bytes/voxel (4 in + 4 out) / step time
+  cpu_baseline  the reference implementation (oracle/_ref, compiled from the
+                reference sources) timed on one host core on the same volume,
+                whose output is also compared bit-for-bit with the GPU's
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "connected-components-3d_amd"), os.path.join(ROOT, "oracle"), ROOT):
+  if p not in sys.path:
+    sys.path.insert(0, p)
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gvoxels/s 512³ 26-conn CCL (bit-exact); batched/served vs plain; p99 latency"
+SHAPE = (512, 512, 512)
+NSEEDS = 12000
+SEED = 512
+
+
+def _profile_rows(lib):
+  names = ctypes.create_string_buffer(64 * 64)
+  ms = (ctypes.c_double * 64)()
+  cnt = (ctypes.c_int * 64)()
+  n = lib.cc3d_amd_profile_collect(names, 64, ms, cnt, 64)
+  rows = {}
+  for i in range(n):
+    nm = names.raw[i * 64:(i + 1) * 64].split(b"\0", 1)[0].decode()
+    rows[nm] = (ms[i], cnt[i])
+  return rows
+
+
+def _traffic_from_profiles(kernel: str):
+  """HBM bytes per launch of `kernel` from the committed counter summary, if any."""
+  path = os.path.join(ROOT, "profiles", "traffic.json")
+  try:
+    with open(path) as f:
+      d = json.load(f)
+    return d.get(kernel, {}).get("bytes_per_launch")
+  except Exception:
+    return None
+
+
+def main() -> int:
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=20)
+  ap.add_argument("--warmup", type=int, default=3)
+  ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+          "(volumes are independent; see the sharding notes in DESIGN.md)", file=sys.stderr)
+    return 2
+
+  import torch
+  import cc3d
+  import synth
+  from cc3d import _lib
+
+  assert torch.cuda.is_available(), "needs a GPU"
+  dev = torch.device("cuda:0")
+  t0 = time.time()
+  vol = synth.voronoi(SHAPE, NSEEDS, seed=SEED)
+  gen_s = time.time() - t0
+  V = vol.size
+  flat = torch.from_numpy(np.ravel(vol, order="F")).to(dev)
+  sx, sy, sz = SHAPE
+  dvol = flat.as_strided(SHAPE, (1, sx, sx * sy))
+  torch.cuda.synchronize()
+
+  def step():
+    return cc3d.connected_components(dvol, connectivity=26, out_dtype=np.uint32, return_N=True)
+
+  for _ in range(args.warmup):
+    out, N = step()
+  torch.cuda.synchronize()
+
+  lib = _lib.lib()
+  lib.cc3d_amd_profile_enable.argtypes = [ctypes.c_int]
+  lib.cc3d_amd_profile_collect.restype = ctypes.c_int
+  lib.cc3d_amd_profile_collect.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+  _profile_rows(lib)  # drop anything recorded before
+  lib.cc3d_amd_profile_enable(1)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    out, N = step()
+  torch.cuda.synchronize()
+  elapsed = time.perf_counter() - t0
+  lib.cc3d_amd_profile_enable(0)
+  rows = _profile_rows(lib)
+
+  ms_per_step = elapsed * 1e3 / args.steps
+  gvox = V * args.steps / elapsed / 1e9
+
+  # dominant kernel and its algorithmic bytes per launch
+  per_kernel = {k: {"ms_per_launch": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rows.items()}
+  algo = {  # bytes each kernel must move per voxel at minimum
+    "k_local": 4.0,      # read the uint32 input once
+    "k_finalize": 4.0,   # write the uint32 output once
+  }
+  dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_launch"]) if per_kernel else "k_local"
+  dom_ms = per_kernel.get(dom, {}).get("ms_per_launch", float("nan"))
+  dom_bytes = algo.get(dom, 0.0) * V
+  achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms and dom_bytes else None
+  roofline = {
+    "bound": "hbm",
+    "kernel": dom,
+    "achieved": achieved,
+    "peak": HBM_PEAK_GBS,
+    "unit": "GB/s",
+    "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+    "traffic": _traffic_from_profiles(dom),
+    "algorithmic_bytes_per_launch": dom_bytes,
+  }
+  path_gbs = 8.0 * V / (ms_per_step * 1e-3) / 1e9
+  kernel_sum_ms = sum(v["ms_per_launch"] for v in per_kernel.values())
+  path_roofline = {
+    "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": path_gbs / HBM_PEAK_GBS,
+    "algorithmic_bytes_per_voxel": 8, "kernel_ms_sum": kernel_sum_ms,
+    "kernel_frac": (8.0 * V / (kernel_sum_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if kernel_sum_ms else None,
+  }
+
+  cpu = None
+  exact = None
+  if not args.no_cpu:
+    import oracle
+    if oracle.have_ref():
+      t0 = time.perf_counter()
+      ref_out, refN = oracle.connected_components(vol, connectivity=26, out_dtype=np.uint32, return_N=True,
+                                                  backend="ref")
+      cpu_s = time.perf_counter() - t0
+      got = out.cpu().numpy()
+      exact = bool(refN == N and np.array_equal(got, ref_out))
+      cpu = {"value": V / cpu_s / 1e9, "unit": "Gvoxels/s", "cores": 1, "kind": "reference",
+             "sample": f"one full 512^3 volume of the workload ({cpu_s:.2f} s single-threaded, "
+                       "estimate_provisional_labels + connected_components3d)"}
+
+  line = {
+    "metric": METRIC,
+    "value": gvox,
+    "unit": "Gvoxels/s",
+    "n_gpus": 1,
+    "steps": args.steps,
+    "warmup": args.warmup,
+    "ms_per_step": ms_per_step,
+    "higher_is_better": True,
+    "scaling": "weak",
+    "vs_baseline": None,
+    "dtype": "u32",
+    "data": "synthetic",
+    "config": {"workload": "voronoi-512^3-uint32-26conn-multilabel", "shape": list(SHAPE), "seeds": NSEEDS,
+               "membrane_p": 0.8, "salt_p": 0.02, "out_dtype": "uint32", "return_N": True,
+               "components": int(N), "input_resident_in_hbm": True},
+    "roofline": roofline,
+    "path_roofline": path_roofline,
+    "kernels": per_kernel,
+    "cpu_baseline": cpu,
+    "bit_exact_vs_reference": exact,
+    "generate_s": gen_s,
+  }
+  print(json.dumps(line))
+  return 0
+
+
+if __name__ == "__main__":
+  sys.exit(main())

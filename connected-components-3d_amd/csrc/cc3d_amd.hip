@@ -22,7 +22,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/cc3d_amd.h"
 #include "common.hpp"
@@ -598,6 +601,47 @@ static int fail(int code, const std::string& msg) {
     if (_e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
   } while (0)
 
+// Optional per-kernel timing with HIP events on the launch stream (enabled by
+// cc3d_amd_profile_enable; read back by cc3d_amd_profile_collect).
+struct ProfRec {
+  hipEvent_t a, b;
+  const char* name;
+};
+static std::vector<ProfRec> g_prof_recs;
+static std::vector<hipEvent_t> g_prof_pool;
+static bool g_prof_on = false;
+static std::mutex g_prof_mu;
+
+static hipEvent_t prof_event() {
+  if (!g_prof_pool.empty()) {
+    hipEvent_t e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  hipEvent_t a = nullptr;
+  hipStream_t s;
+  const char* name;
+  ProfScope(const char* n, hipStream_t st) : s(st), name(n) {
+    if (!g_prof_on) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    a = prof_event();
+    (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    hipEvent_t b = prof_event();
+    (void)hipEventRecord(b, s);
+    g_prof_recs.push_back({a, b, name});
+  }
+};
+
 static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 static inline unsigned grid_for(int64_t items, int64_t per_block, int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, cap));
@@ -715,8 +759,11 @@ int run_global(cc3d_amd_job* j) {
   const GlobalParams& p = j->gp;
   hipStream_t s = j->stream;
   const unsigned wave_blocks = grid_for(j->g.ntiles, 4, 65536);
+  {
+  ProfScope ps_("k_requests", s);
   if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
   else hipLaunchKernelGGL((k_requests<G, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  }
   HIPCHECK(hipGetLastError());
   if (j->periodic && j->g.mode != M_CONT) {
     // a 6-connected single plane wraps like the 4-neighbourhood (its z wrap pairs a voxel with itself)
@@ -725,6 +772,7 @@ int run_global(cc3d_amd_job* j) {
                            : nbp == 4 ? (j->g.sx + j->g.sy) : (3 * j->g.sx + 2 + 3 * j->g.sy);
     const dim3 grid(grid_for(npairs, WG, 65536));
     const int sr = j->special_row_rule;
+    ProfScope ps_("k_periodic", s);
 #define CC3D_P(T)                                                                                             \
   do {                                                                                                        \
     const T* in = static_cast<const T*>(j->in);                                                               \
@@ -749,11 +797,18 @@ int run_global(cc3d_amd_job* j) {
 #undef CC3D_P
     HIPCHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL((k_roots<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  {
+    ProfScope ps_("k_roots", s);
+    hipLaunchKernelGGL((k_roots<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  }
   HIPCHECK(hipGetLastError());
+  {
+  ProfScope ps_("k_scan", s);
   hipLaunchKernelGGL(k_scan_a, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
   hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)j->g.nblk, p.ctr);
   hipLaunchKernelGGL(k_scan_c, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
+  }
+  ProfScope ps_l("k_labels", s);
   hipLaunchKernelGGL((k_labels<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
@@ -896,7 +951,10 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
   unsigned long long* mm = reinterpret_cast<unsigned long long*>(ws + g.oMM);
   hipStream_t s = j->stream;
 
-  hipLaunchKernelGGL(k_init, dim3(grid_for(g.nwords, WG, 8192)), dim3(WG), 0, s, gp.ctr, gp.bits, g.nwords);
+  {
+    ProfScope ps_("k_init", s);
+    hipLaunchKernelGGL(k_init, dim3(grid_for(g.nwords, WG, 8192)), dim3(WG), 0, s, gp.ctr, gp.bits, g.nwords);
+  }
   if (hipGetLastError() != hipSuccess) {
     cc3d_amd_job_free(j);
     return fail(CC3D_AMD_ERR_HIP, "k_init launch failed");
@@ -944,9 +1002,12 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
   lp.TQ = reinterpret_cast<uint32_t*>(ws + g.oTQ);
   lp.ctr = gp.ctr;
   hipError_t e;
+  {
+  ProfScope ps_("k_local", s);
   if (g.mode == M_MULTI) e = launch_local_multi(in_dtype, g.nb, g.qk, lp, s);
   else if (g.mode == M_BINARY) e = launch_local_binary(in_dtype, g.nb, g.qk, lp, s);
   else e = launch_local_cont(in_dtype, g.nb, g.qk, lp, s);
+  }
   if (e != hipSuccess) {
     cc3d_amd_job_free(j);
     return fail(CC3D_AMD_ERR_HIP, std::string("k_local: ") + hipGetErrorString(e));
@@ -991,6 +1052,7 @@ int cc3d_amd_label_finish(cc3d_amd_job* j, void* out, int out_dtype) {
   hipStream_t s = j->stream;
   const dim3 grid((unsigned)g.ntiles), block(WG);
   int rc = CC3D_AMD_OK;
+  ProfScope ps_("k_finalize", s);
 #define CC3D_F(OUT)                                                                                \
   do {                                                                                             \
     if (g.is3d) hipLaunchKernelGGL((k_finalize<OUT, Geo3>), grid, block, 0, s, j->gp, (OUT*)out); \
@@ -1119,6 +1181,39 @@ int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, in
   cc3d_amd_job_free(j);
   if (rc == CC3D_AMD_OK) HIPCHECK(hipStreamSynchronize(s));
   return rc;
+}
+
+void cc3d_amd_profile_enable(int on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_on = on != 0;
+}
+
+// Waits for the recorded events and returns up to cap (name, total ms, count)
+// rows aggregated by kernel name, clearing the records.  Returns the row count.
+int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* counts, int cap) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  std::map<std::string, std::pair<double, int>> agg;
+  for (auto& r : g_prof_recs) {
+    (void)hipEventSynchronize(r.b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto& e = agg[r.name];
+    e.first += ms;
+    e.second += 1;
+    g_prof_pool.push_back(r.a);
+    g_prof_pool.push_back(r.b);
+  }
+  g_prof_recs.clear();
+  int n = 0;
+  for (auto& kv : agg) {
+    if (n >= cap) break;
+    std::strncpy(names + (size_t)n * name_len, kv.first.c_str(), name_len - 1);
+    names[(size_t)n * name_len + name_len - 1] = 0;
+    total_ms[n] = kv.second.first;
+    counts[n] = kv.second.second;
+    n++;
+  }
+  return n;
 }
 
 }  // extern "C"

@@ -73,6 +73,37 @@ __device__ inline bool in_box(int bx, int by, int bz) {
   return bx >= 0 && bx < G::BX && by >= 0 && by < G::BY && bz >= 0 && bz < G::BZ;
 }
 
+// Raster order of two box positions (z, then y, then x).
+__device__ inline bool box_later(int ax, int ay, int az, int bx, int by, int bz) {
+  if (az != bz) return az > bz;
+  if (ay != by) return ay > by;
+  return ax > bx;
+}
+
+// Is the adjacent pair (a, b) unioned inside this tile's structure: by the
+// tile phase (its later voxel is a tile voxel) or by the halo phase (both are
+// halo voxels of the same neighbouring tile)?
+template <class G>
+__device__ inline bool pair_here(int ax, int ay, int az, int bx, int by, int bz) {
+  const bool ta = G::is_tile(ax, ay, az), tb = G::is_tile(bx, by, bz);
+  if (box_later(ax, ay, az, bx, by, bz) ? ta : tb) return true;
+  return !ta && !tb && tile_slot<G>(ax, ay, az) == tile_slot<G>(bx, by, bz);
+}
+
+// All backward neighbours of this tile voxel lie inside the tile.
+template <class G, int NB>
+__device__ inline bool interior_voxel(int bx, int by, int bz) {
+  if constexpr (NB == 26 || NB == 18)
+    return bx >= 2 && bx <= G::TX - 1 && by >= 2 && by <= G::TY - 1 && bz >= 2;
+  else if constexpr (NB == 6)
+    return bx >= 2 && by >= 2 && bz >= 2;
+  else if constexpr (NB == 8)
+    return bx >= 2 && bx <= G::TX - 1 && by >= 2;
+  else
+    return bx >= 2 && by >= 2;
+}
+
+
 // Unions for one foreground voxel at box (bx,by,bz).  HALO: restrict to halo
 // neighbours in the same neighbouring tile.  Returns the number of linked
 // neighbours found (0 <=> the reference gives this voxel a new label).
@@ -147,6 +178,11 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int bx, int by, int
       const T q = val(qx, qy, qz);
       return MODE == M_MULTI ? (q == cur) : fg(q);
     };
+    // a phantom path a-c-b connects a and b when both of its pairs are
+    // processed in this tile's structure
+    auto path = [&](int ax, int ay, int az, int cx, int cy, int cz, int qx, int qy, int qz) -> bool {
+      return HALO || (pair_here<G>(ax, ay, az, cx, cy, cz) && pair_here<G>(cx, cy, cz, qx, qy, qz));
+    };
     const bool lM = lk(bx - 1, by, bz);
     const bool lK = lk(bx, by - 1, bz);
     int n = 0;
@@ -155,62 +191,89 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int bx, int by, int
       n++;
     }
     if (lK) {
-      if (!(lM && lk(bx - 1, by - 1, bz))) lunion(par, pn, nd(bx, by - 1, bz));
+      if (!(lM && lk(bx - 1, by - 1, bz) && path(bx - 1, by, bz, bx - 1, by - 1, bz, bx, by - 1, bz)))
+        lunion(par, pn, nd(bx, by - 1, bz));
       n++;
     }
     if constexpr (NB == 6) {
       if (lk(bx, by, bz - 1)) {
-        if (!((lM && lk(bx - 1, by, bz - 1)) || (lK && lk(bx, by - 1, bz - 1))))
-          lunion(par, pn, nd(bx, by, bz - 1));
+        const bool viaD = lM && lk(bx - 1, by, bz - 1) && path(bx - 1, by, bz, bx - 1, by, bz - 1, bx, by, bz - 1);
+        const bool viaB = lK && lk(bx, by - 1, bz - 1) && path(bx, by - 1, bz, bx, by - 1, bz - 1, bx, by, bz - 1);
+        if (!(viaD || viaB)) lunion(par, pn, nd(bx, by, bz - 1));
         n++;
       }
     }
     return n;
-  } else if constexpr (MODE != M_CONT) {
-    // 26 / 18 / 8: transitive links, adjacency covering
-    uint32_t cov = 0;
-    int n = 0;
-#pragma unroll
-    for (int i = 0; i < NH::N; i++) {
-      if ((cov >> i) & 1u) continue;
-      const Off o = NH::off(i);
-      const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
-      if (!ok(qx, qy, qz)) continue;
-      const T q = val(qx, qy, qz);
-      const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
-      if (!l) continue;
-      n++;
-      lunion(par, pn, nd(qx, qy, qz));
-      cov |= NH::adj(i);
-    }
-    return n;
   } else {
-    // continuous: exact-equality covering plus direct matches with neighbour 0
-    uint32_t cov = 0;
-    int n = 0;
-    bool hub = false;
-    T hv = T(0);
+    // Covering masks.  Interior voxels (and the halo phase, where every pair is
+    // two halo voxels of one neighbouring tile) use plain adjacency.  On a tile
+    // border a neighbour j may cover i only when the pair (j, i) is itself
+    // unioned in this tile's structure: the later voxel of the pair lies in the
+    // tile, or both are halo voxels of the same neighbouring tile.  Otherwise
+    // the tiles' local forests could lean on each other's cross-tile links.
+    const bool interior = HALO || interior_voxel<G, NB>(bx, by, bz);
+    auto cover_from = [&](int j) -> uint32_t {
+      uint32_t m = NH::adj(j);
+      if (interior) return m;
+      const Off oj = NH::off(j);
+      uint32_t r = 0;
 #pragma unroll
-    for (int i = 0; i < NH::N; i++) {
-      if ((cov >> i) & 1u) continue;
-      const Off o = NH::off(i);
-      const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
-      if (!ok(qx, qy, qz)) continue;
-      const T q = val(qx, qy, qz);
-      bool l = fg(q) && match(cur, q, delta);
-      if (NB == 26 && i == 0 && q == cur) l = true;
-      if (!l) continue;
-      n++;
-      bool covered = false;
-      if (i > 0 && hub && ((NH::adj(0) >> i) & 1u) && match(hv, q, delta)) covered = true;
-      if (!covered) lunion(par, pn, nd(qx, qy, qz));
-      if (i == 0) {
-        hub = true;
-        hv = q;
+      for (int i = 0; i < NH::N; i++) {
+        if (!((m >> i) & 1u)) continue;
+        const Off oi = NH::off(i);
+        if (pair_here<G>(bx + oj.dx, by + oj.dy, bz + oj.dz, bx + oi.dx, by + oi.dy, bz + oi.dz)) r |= 1u << i;
       }
-      if (q == cur) cov |= NH::adj(i);
+      return r;
+    };
+    if constexpr (MODE != M_CONT) {
+      // 26 / 18 / 8: transitive links
+      uint32_t cov = 0;
+      int n = 0;
+#pragma unroll
+      for (int i = 0; i < NH::N; i++) {
+        if ((cov >> i) & 1u) continue;
+        const Off o = NH::off(i);
+        const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
+        if (!ok(qx, qy, qz)) continue;
+        const T q = val(qx, qy, qz);
+        const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
+        if (!l) continue;
+        n++;
+        lunion(par, pn, nd(qx, qy, qz));
+        cov |= cover_from(i);
+      }
+      return n;
+    } else {
+      // continuous: exact-equality covering plus direct matches with neighbour 0
+      uint32_t cov = 0;
+      int n = 0;
+      bool hub = false;
+      uint32_t hubcov = 0;
+      T hv = T(0);
+#pragma unroll
+      for (int i = 0; i < NH::N; i++) {
+        if ((cov >> i) & 1u) continue;
+        const Off o = NH::off(i);
+        const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
+        if (!ok(qx, qy, qz)) continue;
+        const T q = val(qx, qy, qz);
+        bool l = fg(q) && match(cur, q, delta);
+        if (NB == 26 && i == 0 && q == cur) l = true;
+        if (!l) continue;
+        n++;
+        bool covered = false;
+        if (i > 0 && hub && ((hubcov >> i) & 1u) && match(hv, q, delta)) covered = true;
+        if (!covered) lunion(par, pn, nd(qx, qy, qz));
+        const uint32_t cf = cover_from(i);
+        if (i == 0) {
+          hub = true;
+          hv = q;
+          hubcov = cf;
+        }
+        if (q == cur) cov |= cf;
+      }
+      return n;
     }
-    return n;
   }
 }
 

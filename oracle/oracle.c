@@ -53,6 +53,13 @@
 enum { DT_U8 = 0, DT_U16 = 1, DT_U32 = 2, DT_U64 = 3, DT_F32 = 4, DT_F64 = 5 };
 enum { M_MULTI = 0, M_BINARY = 1, M_CONT = 2 };
 
+/* Deliberate mistakes, for tests that show the checks catch them:
+ * 1 drop the 26-connected corner neighbours, 2 number components by their
+ * last voxel, 3 ignore the binary 2D-4 row-0 equality rule, 4 ignore the
+ * continuous 2D-8 range shortcut, 5 skip the stale binary 2D-8 labels. */
+static int g_mutation = 0;
+void oracle_set_mutation(int m) { g_mutation = m; }
+
 /* ---------------- union-find (plain, sequential) ---------------- */
 static uint32_t uf_find(uint32_t* P, uint32_t a) {
   while (P[a] != a) {
@@ -369,7 +376,8 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
         if (q2) {
           /* cc3d/cc3d_binary.hpp:822-935 */
           if (y == 0) {
-            if (x > 0 && is_eq(&v, p, p - 1)) { uf_union(par, p, p - 1); nlinks++; }
+            const int lk = x > 0 && (g_mutation == 3 ? is_fg(&v, p - 1) : is_eq(&v, p, p - 1));
+            if (lk) { uf_union(par, p, p - 1); nlinks++; }
           } else {
             const int B = x > 0 && is_fg(&v, p - 1);
             const int C = is_fg(&v, p - sx);
@@ -382,7 +390,7 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
           if (y > 0 && is_eq(&v, p, p - sx)) {
             uf_union(par, p, p - sx);
             nlinks++;
-          } else if (y > 0 && is_fg(&v, p - sx) &&
+          } else if (g_mutation != 4 && y > 0 && is_fg(&v, p - sx) &&
                      q3_span(&v, p, p - sx, delta_u, delta_f, gmin_f, gmax_f, gmin_u, gmax_u)) {
             uf_union(par, p, p - sx);
             nlinks++;
@@ -401,6 +409,7 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
           for (int k = 0; k < noff; k++) {
             const int64_t qx = x + offs[k].dx, qy = y + offs[k].dy, qz = z + offs[k].dz;
             if (qx < 0 || qx >= sx || qy < 0 || qy >= sy || qz < 0) continue;
+            if (g_mutation == 1 && offs[k].dx && offs[k].dy && offs[k].dz) continue;
             const int64_t q = qx + sx * (qy + sy * qz);
             int lk;
             if (mode == M_MULTI) lk = is_eq(&v, p, q);
@@ -477,6 +486,7 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
       key = (uint32_t)p;
     }
     uint32_t r = uf_find(par, (uint32_t)p);
+    if (g_mutation == 2) key = (uint32_t)(K - 1 - key);
     if (key < minkey[r]) minkey[r] = key;
   }
   for (int64_t p = 0; p < V; p++)
@@ -490,7 +500,7 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
     out[p] = is_fg(&v, p) ? rank[minkey[uf_find(par, (uint32_t)p)]] : 0;
   *N = n;
 
-  if (q1) {
+  if (q1 && g_mutation != 5) {
     /* provisional block labels in block raster order (for P and Q6) */
     const int64_t nb = osx * osy;
     uint64_t* bl = (uint64_t*)calloc(nb, sizeof(uint64_t));

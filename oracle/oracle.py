@@ -56,6 +56,8 @@ def _lib_oracle():
       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
       ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
     ]
+    lib.oracle_set_mutation.restype = None
+    lib.oracle_set_mutation.argtypes = [ctypes.c_int]
     lib.oracle_epl.restype = ctypes.c_uint64
     lib.oracle_epl.argtypes = [
       ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
@@ -108,6 +110,11 @@ def core_label(flat: np.ndarray, sx: int, sy: int, sz: int, connectivity: int,
   if rc != 0:
     raise RuntimeError("oracle_label failed")
   return out, int(N.value), int(P.value)
+
+
+def set_mutation(m: int) -> None:
+  """Plant a known mistake in the oracle (0 = none); see oracle.c."""
+  _lib_oracle().oracle_set_mutation(int(m))
 
 
 def core_epl(flat: np.ndarray, sx: int):

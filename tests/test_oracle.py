@@ -1,0 +1,242 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the GPU checker.
+
+Checks, all on the CPU:
+  * the committed reference vectors (tests/golden/ref_vectors.npz, produced by
+    the reference compiled from its own sources);
+  * the compiled reference itself on fresh seeded inputs, when oracle/_ref is
+    present;
+  * scipy.ndimage.label on binary volumes (the reference's own cross-check,
+    automated_test.py:499-554);
+  * brute-force breadth-first labelling on tiny multilabel volumes;
+  * estimate_provisional_labels against a direct numpy count;
+  * planted mistakes in the oracle are caught by the golden vectors.
+"""
+import os
+import sys
+from collections import deque
+
+import numpy as np
+import pytest
+
+import oracle
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import make_golden  # noqa: E402
+
+
+def _compare(arr, kw, exp_out, expN, err):
+  try:
+    out, N = oracle.connected_components(arr, return_N=True, backend="oracle", **kw)
+  except Exception as e:
+    assert err and type(e).__name__ == err, f"oracle raised {e!r}, reference {err or 'ok'}"
+    return
+  assert not err, f"oracle ok, reference raised {err}"
+  assert N == expN
+  assert out.dtype == exp_out.dtype and out.shape == exp_out.shape
+  assert np.array_equal(out, exp_out)
+
+
+def test_oracle_matches_reference_vectors():
+  for arr, kw, out, N, err in make_golden.load():
+    _compare(arr, kw, out, N, err)
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built")
+def test_oracle_matches_compiled_reference_random():
+  rng = np.random.default_rng(99)
+  for i in range(300):
+    arr, order, kw = make_golden.case(rng, i)
+    kw2 = dict(kw)
+    if "out_dtype" in kw2:
+      kw2["out_dtype"] = np.dtype(kw2["out_dtype"])
+    try:
+      ref = oracle.connected_components(arr, return_N=True, backend="ref", **kw2)
+      _compare(arr, kw2, ref[0], ref[1], "")
+    except RuntimeError:
+      with pytest.raises(RuntimeError):
+        oracle.connected_components(arr, return_N=True, backend="oracle", **kw2)
+
+
+@pytest.mark.parametrize("conn", [26, 18, 6])
+def test_binary_3d_matches_scipy(conn):
+  import scipy.ndimage
+  rng = np.random.default_rng(conn)
+  labels = rng.integers(0, 2, (40, 37, 33)).astype(bool)
+  if conn == 26:
+    st = np.ones((3, 3, 3))
+  elif conn == 18:
+    st = np.array([[[0, 1, 0], [1, 1, 1], [0, 1, 0]], [[1, 1, 1]] * 3, [[0, 1, 0], [1, 1, 1], [0, 1, 0]]])
+  else:
+    st = None
+  ours, N = oracle.connected_components(labels, connectivity=conn, return_N=True)
+  sp, Ns = scipy.ndimage.label(labels, structure=st)
+  assert N == Ns
+  assert np.array_equal(ours, sp)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_binary_2d_partition_matches_scipy(conn):
+  """2D binary partitions equal scipy's (8-conn numbering follows 2x2 blocks)."""
+  import scipy.ndimage
+  rng = np.random.default_rng(conn + 100)
+  img = rng.integers(0, 2, (61, 47)).astype(bool)
+  st = np.ones((3, 3)) if conn == 8 else None
+  ours, N = oracle.connected_components(img, connectivity=conn, return_N=True)
+  sp, Ns = scipy.ndimage.label(img, structure=st)
+  assert N == Ns
+  pairs = set(zip(ours[img].tolist(), sp[img].tolist()))
+  assert len(pairs) == N  # a bijection between the two labelings
+
+
+def _bfs_label(vol, conn):
+  """Textbook BFS labelling, numbered by first appearance in x-fastest order."""
+  sx, sy, sz = vol.shape
+  offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+          if (dx, dy, dz) != (0, 0, 0)]
+  m = {26: 3, 18: 2, 6: 1}[conn]
+  offs = [o for o in offs if sum(map(abs, o)) <= m]
+  lab = np.zeros(vol.shape, dtype=np.int64)
+  n = 0
+  for z in range(sz):
+    for y in range(sy):
+      for x in range(sx):
+        if vol[x, y, z] == 0 or lab[x, y, z]:
+          continue
+        n += 1
+        lab[x, y, z] = n
+        q = deque([(x, y, z)])
+        while q:
+          a = q.popleft()
+          for o in offs:
+            b = (a[0] + o[0], a[1] + o[1], a[2] + o[2])
+            if 0 <= b[0] < sx and 0 <= b[1] < sy and 0 <= b[2] < sz and not lab[b] and vol[b] == vol[a]:
+              lab[b] = n
+              q.append(b)
+  return lab, n
+
+
+@pytest.mark.parametrize("conn", [26, 18, 6])
+def test_multilabel_matches_bfs(conn):
+  rng = np.random.default_rng(7 + conn)
+  for _ in range(6):
+    vol = np.asfortranarray(rng.integers(0, 4, (9, 8, 7)).astype(np.uint16))
+    ours, N = oracle.connected_components(vol, connectivity=conn, return_N=True, out_dtype=np.uint32)
+    exp, n = _bfs_label(vol, conn)
+    assert N == n
+    assert np.array_equal(ours.astype(np.int64), exp)
+
+
+def test_epl_matches_numpy_count():
+  rng = np.random.default_rng(3)
+  for _ in range(10):
+    vol = np.asfortranarray(rng.integers(0, 3, (17, 9, 5)).astype(np.uint32))
+    epl, f, l = oracle.estimate_provisional_labels(vol)
+    flat = np.ravel(vol, order="F").reshape(-1, 17)
+    starts = (flat != 0) & np.concatenate([np.ones((flat.shape[0], 1), bool), flat[:, 1:] != flat[:, :-1]], 1)
+    assert epl == int(starts.sum())
+    rows = np.nonzero(starts.any(1))[0]
+    assert (f, l) == ((int(rows[0]), int(rows[-1])) if len(rows) else (-1, -1))
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4, 5])
+def test_planted_mistakes_are_caught(mutation):
+  """Each planted oracle mistake makes at least one golden vector disagree."""
+  cases = make_golden.load()
+  oracle.set_mutation(mutation)
+  try:
+    caught = False
+    for arr, kw, out, N, err in cases:
+      try:
+        _compare(arr, kw, out, N, err)
+      except AssertionError:
+        caught = True
+        break
+    assert caught, f"mutation {mutation} went unnoticed"
+  finally:
+    oracle.set_mutation(0)
+
+
+# ---- known answers transcribed from the reference's own test suite ---------
+
+def test_known_2d_cross():
+  img = np.ones((17, 17), dtype=np.uint8)
+  img[:, 8] = 0
+  img[8, :] = 0
+  gt = np.zeros((17, 17), dtype=np.uint32)
+  gt[:8, :8], gt[:8, 9:], gt[9:, :8], gt[9:, 9:] = 1, 2, 3, 4
+  for conn in (4, 8, 6, 18, 26):
+    assert np.array_equal(oracle.connected_components(img, connectivity=conn), gt)
+  img[9:, 9:] = 2
+  assert np.array_equal(oracle.connected_components(img, connectivity=26), gt)
+
+
+def test_known_diagonals():
+  inp = np.array([
+    [0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [1, 0, 0, 1, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0],
+    [0, 1, 0, 0, 1, 0, 0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0],
+    [1, 0, 1, 0, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0, 0, 0, 0],
+    [0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0, 0, 0, 0],
+    [0, 0, 1, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0],
+    [0, 1, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0],
+  ], dtype=np.uint32)
+  gt8 = np.array([
+    [0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [2, 0, 0, 1, 0, 0, 3, 0, 4, 4, 0, 0, 0, 0, 0, 0, 0],
+    [0, 2, 0, 0, 1, 0, 0, 0, 4, 4, 0, 0, 0, 0, 0, 0, 0],
+    [2, 0, 2, 0, 0, 0, 4, 4, 0, 0, 4, 4, 0, 0, 0, 0, 0],
+    [0, 0, 0, 0, 0, 0, 4, 4, 0, 0, 4, 4, 0, 0, 0, 0, 0],
+    [0, 0, 5, 0, 6, 0, 0, 0, 0, 4, 0, 0, 0, 0, 0, 0, 0],
+    [0, 5, 0, 0, 0, 6, 0, 0, 4, 0, 0, 0, 0, 0, 0, 0, 0],
+  ], dtype=np.uint32)
+  for conn in (8, 18, 26):
+    assert np.array_equal(oracle.connected_components(inp, connectivity=conn), gt8)
+  gt4 = np.array([
+    [0, 0, 1, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [3, 0, 0, 4, 0, 0, 5, 0, 6, 6, 0, 0, 0, 0, 0, 0, 0],
+    [0, 7, 0, 0, 8, 0, 0, 0, 6, 6, 0, 0, 0, 0, 0, 0, 0],
+    [9, 0, 10, 0, 0, 0, 11, 11, 0, 0, 12, 12, 0, 0, 0, 0, 0],
+    [0, 0, 0, 0, 0, 0, 11, 11, 0, 0, 12, 12, 0, 0, 0, 0, 0],
+    [0, 0, 13, 0, 14, 0, 0, 0, 0, 15, 0, 0, 0, 0, 0, 0, 0],
+    [0, 16, 0, 0, 0, 17, 0, 0, 18, 0, 0, 0, 0, 0, 0, 0, 0],
+  ], dtype=np.uint32)
+  for conn in (4, 6):
+    assert np.array_equal(oracle.connected_components(inp, connectivity=conn), gt4)
+
+
+def test_known_cross_with_intruder():
+  img = np.ones((5, 5), dtype=np.uint8)
+  img[:, 2] = 0
+  img[2, :] = 0
+  img[3:, 3:] = 2
+  img[3, 3] = 1
+  gt = np.array([[1, 1, 0, 2, 2], [1, 1, 0, 2, 2], [0, 0, 0, 0, 0], [3, 3, 0, 4, 5], [3, 3, 0, 5, 5]])
+  for conn in (4, 6, 8, 18, 26):
+    assert np.array_equal(oracle.connected_components(img, connectivity=conn), gt)
+
+
+def test_known_periodic_counts():
+  lab = np.zeros((10, 10), dtype=np.uint32, order="F")
+  lab[:, 0] = 1
+  lab[:, -1] = 1
+  for conn in (4, 8):
+    assert oracle.connected_components(lab, connectivity=conn, return_N=True)[1] == 2
+    assert oracle.connected_components(lab, connectivity=conn, return_N=True, periodic_boundary=True)[1] == 1
+  lab[0, 0], lab[-1, 0], lab[0, -1], lab[-1, -1] = 2, 3, 3, 2
+  assert oracle.connected_components(lab, connectivity=8, return_N=True, periodic_boundary=True)[1] == 3
+  v = np.zeros((10, 10, 10), dtype=np.uint32, order="F")
+  v[1, :, :] = 1
+  v[-2, :, :] = 1
+  assert oracle.connected_components(v, connectivity=6, return_N=True, periodic_boundary=True)[1] == 2
+  v[:] = 0
+  v[0, :, :] = 1
+  v[-1, :, :] = 1
+  assert oracle.connected_components(v, connectivity=6, return_N=True, periodic_boundary=True)[1] == 1
+
+
+def test_known_out_dtypes():
+  assert oracle.connected_components(np.zeros((64, 64, 64), np.uint8)).dtype == np.uint16
+  with pytest.raises(ValueError):
+    oracle.connected_components(np.arange(41 ** 3, dtype=np.uint32) + 1, out_dtype=np.uint16)
+  with pytest.raises(ValueError):
+    oracle.connected_components(np.ones((4, 4, 4), np.uint8), out_dtype=np.uint8)
