@@ -1,0 +1,121 @@
+"""statistics and dust on the GPU against CPU restatements.
+
+statistics is checked against numpy (bincount for counts, per-label min/max of
+np.nonzero coordinates, sums / counts for centroids -- exact integer sums, so
+the float64 division must match bit for bit).  dust is checked against a
+restatement of cc3d/__init__.py:71-155 driven by the oracle's labels.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _np_stats(labels):
+  lab = labels
+  while lab.ndim < 2:
+    lab = lab[..., None]
+  N = int(lab.max())
+  nd = lab.ndim
+  counts = np.bincount(lab.ravel().astype(np.int64), minlength=N + 1).astype(np.uint32)
+  idx = np.nonzero(np.ones(lab.shape, bool))
+  flat = lab.ravel()  # C order, matching idx
+  bb = np.zeros((N + 1, 2 * nd), dtype=np.int64)
+  cen = np.full((N + 1, nd), np.nan)
+  for a in range(nd):
+    mn = np.full(N + 1, np.iinfo(np.uint16).max if max(lab.shape) <= 65535 else np.iinfo(np.uint32).max, np.int64)
+    mx = np.zeros(N + 1, np.int64)
+    np.minimum.at(mn, flat, idx[a])
+    np.maximum.at(mx, flat, idx[a])
+    bb[:, 2 * a], bb[:, 2 * a + 1] = mn, mx
+    s = np.zeros(N + 1, np.int64)
+    np.add.at(s, flat, idx[a])
+    with np.errstate(invalid="ignore", divide="ignore"):
+      c = s.astype(np.float64) / counts.astype(np.float64)
+    cen[:, a] = np.where(counts == 0, np.nan, c)
+  return counts, bb, cen
+
+
+@pytest.mark.parametrize("order", ["C", "F"])
+@pytest.mark.parametrize("shape", [(40, 33, 21), (70, 45), (200,)])
+def test_statistics_match_numpy(gpu, order, shape):
+  import cc3d
+  rng = np.random.default_rng(len(shape))
+  lab = rng.integers(0, 30, size=shape).astype(np.uint32)
+  lab[lab == 7] = 0  # an absent label
+  lab = np.asfortranarray(lab) if order == "F" else np.ascontiguousarray(lab)
+  st = cc3d.statistics(lab, no_slice_conversion=True)
+  counts, bb, cen = _np_stats(lab)
+  assert np.array_equal(st["voxel_counts"], counts)
+  assert np.array_equal(st["bounding_boxes"].astype(np.int64), bb)
+  assert np.array_equal(np.isnan(st["centroids"]), np.isnan(cen))
+  assert np.array_equal(np.nan_to_num(st["centroids"]), np.nan_to_num(cen))
+
+
+def test_statistics_reference_known_answer(gpu):
+  """cc3d's own statistics test values (automated_test.py:962-1006)."""
+  import cc3d
+  for order in ("C", "F"):
+    labels = np.zeros((123, 128, 125), dtype=np.uint8, order=order)
+    labels[10:20, 10:20, 10:20] = 1
+    labels[40:50, 40:50, 40:51] = 2
+    st = cc3d.statistics(labels)
+    assert st["voxel_counts"][1] == 1000
+    assert st["voxel_counts"][2] == 10 * 10 * 11
+    assert np.allclose(st["centroids"][1, :], [14.5, 14.5, 14.5])
+    assert np.allclose(st["centroids"][2, :], [44.5, 44.5, 45])
+    assert st["bounding_boxes"][0] == (slice(0, 123), slice(0, 128), slice(0, 125))
+    assert st["bounding_boxes"][1] == (slice(10, 20), slice(10, 20), slice(10, 20))
+    assert st["bounding_boxes"][2] == (slice(40, 50), slice(40, 50), slice(40, 51))
+  st = cc3d.statistics(np.zeros((0, 1, 1), dtype=np.uint8))
+  assert st == {"voxel_counts": None, "bounding_boxes": None, "centroids": None}
+
+
+def _dust_restated(img, threshold, connectivity, invert):
+  """cc3d/__init__.py:71-155 with the oracle's labels."""
+  orig = img.dtype
+  img = img.view(f"u{img.dtype.itemsize}") if img.dtype.kind == "i" else img
+  img = np.copy(img)
+  cc, N = oracle.connected_components(img, connectivity=connectivity, return_N=True)
+  sizes = np.bincount(cc.ravel().astype(np.int64), minlength=N + 1)
+  if isinstance(threshold, (tuple, list)):
+    to_mask = [i for i in range(1, N + 1) if not (threshold[0] <= sizes[i] < threshold[1])]
+  else:
+    to_mask = [i for i in range(1, N + 1) if sizes[i] < threshold]
+  dust_N = len(to_mask) if invert else N - len(to_mask)
+  if not to_mask:
+    if invert:
+      img = np.zeros(img.shape, dtype=img.dtype, order="F")
+    return img, dust_N
+  mask = np.isin(cc, to_mask, invert=invert)
+  img[mask] = 0
+  return img.view(orig), dust_N
+
+
+@pytest.mark.parametrize("invert", [False, True])
+@pytest.mark.parametrize("threshold", [20, (25, 491)])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32])
+def test_dust_matches_restatement(gpu, invert, threshold, dtype):
+  import cc3d
+  rng = np.random.default_rng(5)
+  img = rng.integers(0, 5, size=(60, 50, 40)).astype(dtype)
+  got, gN = cc3d.dust(img, threshold=threshold, connectivity=26, invert=invert, return_N=True)
+  exp, eN = _dust_restated(img, threshold, 26, invert)
+  assert gN == eN
+  assert got.dtype == exp.dtype
+  assert np.array_equal(got, exp)
+
+
+def test_dust_reference_known_answer(gpu):
+  """automated_test.py:1160-1200 (dust static)."""
+  import cc3d
+  for conn in (6, 18, 26):
+    labels = np.zeros((100, 100, 10), dtype=np.uint8, order="F")
+    labels[:5, :5, :1] = 1
+    labels[20:40, 20:40, :] = 2
+    rec, N = cc3d.dust(labels, threshold=26, connectivity=conn, return_N=True)
+    assert list(np.unique(rec)) == [0, 2] and N == 1
+    rec, N = cc3d.dust(labels, threshold=26, connectivity=conn, invert=True, return_N=True)
+    assert list(np.unique(rec)) == [0, 1] and N == 1
