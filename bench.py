@@ -72,6 +72,8 @@ def main() -> int:
   ap.add_argument("--steps", type=int, default=20)
   ap.add_argument("--warmup", type=int, default=3)
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+  ap.add_argument("--calib", action="store_true",
+                  help="also run one 256 MiB device copy (counter calibration for profiles)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -93,6 +95,13 @@ def main() -> int:
   sx, sy, sz = SHAPE
   dvol = flat.as_strided(SHAPE, (1, sx, sx * sy))
   torch.cuda.synchronize()
+
+  if args.calib:
+    a = torch.empty(64 << 20, dtype=torch.int32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
   def step():
     return cc3d.connected_components(dvol, connectivity=26, out_dtype=np.uint32, return_N=True)

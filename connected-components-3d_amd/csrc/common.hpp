@@ -53,6 +53,8 @@ struct TileGeo {
   static constexpr int HALO = IS3D ? BX * BY + TZ * RING : BX + 2 * TY;
   static constexpr int NN = TV + HALO;
   static_assert(TV % WG == 0, "tile must be a multiple of the workgroup");
+  static_assert(IS3D ? (TX * TY == WG && TZ == PER_THREAD) : (TX * TY == WG * PER_THREAD && TX <= WG),
+                "one column of PER_THREAD voxels per thread");
   static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0 && (TZ & (TZ - 1)) == 0,
                 "tile sides are powers of two");
 
@@ -107,9 +109,26 @@ struct TileGeo {
       bx = (h & 1) ? BX - 1 : 0;
     }
   }
-  // Position of a tile voxel inside its tile (raster order); also the index
-  // used for the per-voxel tables J8/J16.
+  // Position of a tile voxel inside its tile (raster order).
   __device__ static inline int pos(int lx, int ly, int lz) { return lx + TX * (ly + TY * lz); }
+  // Thread t owns a column of PER_THREAD voxels along the column axis (z in
+  // 3D, y in 2D); voxel k of the column.
+  __device__ static inline void voxel(int t, int k, int& lx, int& ly, int& lz) {
+    lx = t & (TX - 1);
+    if (IS3D) {
+      ly = t / TX;
+      lz = k;
+    } else {
+      ly = (t / TX) * PER_THREAD + k;
+      lz = 0;
+    }
+  }
+  // Index of a tile voxel in the per-voxel tables J8/J16: thread-major, so a
+  // thread's column is PER_THREAD consecutive entries.
+  __device__ static inline int jidx(int lx, int ly, int lz) {
+    if (IS3D) return (lx + TX * ly) * PER_THREAD + lz;
+    return (lx + TX * (ly / PER_THREAD)) * PER_THREAD + (ly & (PER_THREAD - 1));
+  }
   // Union-find node of a tile voxel: its raster position, or the 2x2 block
   // order used by the binary 2D 8-connected numbering.
   template <int QK>
@@ -192,12 +211,36 @@ __device__ __host__ inline T from_bits(uint64_t b) {
 // ---------------------------------------------------------------------------
 // union-find
 // ---------------------------------------------------------------------------
-// LDS: parents are node ids; the root of a tree is its minimum node id.
-__device__ inline uint32_t lfind(volatile uint32_t* par, uint32_t a) {
-  uint32_t p = par[a];
+// LDS: parents are node ids; the root of a tree is its minimum node id and
+// every parent pointer points to a smaller id.
+//
+// lfind_h halves the path while other lanes are still unioning: each write is
+// an atomicMin to the grandparent, an ancestor in the same tree, so it can
+// never undo another lane's link (parents only ever decrease).
+__device__ inline uint32_t lload(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline uint32_t lfind_h(uint32_t* par, uint32_t a) {
+  uint32_t p = lload(par + a);
   while (p != a) {
+    const uint32_t gp = lload(par + p);
+    if (gp == p) return p;
+    atomicMin(&par[a], gp);
+    a = gp;
+    p = lload(par + a);
+  }
+  return a;
+}
+
+// Plain find for the read-only phase after all unions (path halving with
+// plain stores: every concurrent writer stores an ancestor).
+__device__ inline uint32_t lfind(uint32_t* par, uint32_t a) {
+  uint32_t p = lload(par + a);
+  while (p != a) {
+    const uint32_t gp = lload(par + p);
+    if (gp != p) __hip_atomic_store(par + a, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     a = p;
-    p = par[a];
+    p = gp;
   }
   return a;
 }
@@ -206,8 +249,8 @@ __device__ inline uint32_t lfind(volatile uint32_t* par, uint32_t a) {
 // retry from the displaced parent when another lane got there first.
 __device__ inline void lunion(uint32_t* par, uint32_t a, uint32_t b) {
   while (true) {
-    a = lfind(par, a);
-    b = lfind(par, b);
+    a = lfind_h(par, a);
+    b = lfind_h(par, b);
     if (a == b) return;
     if (a < b) {
       const uint32_t t = a;
@@ -225,10 +268,13 @@ __device__ inline void lunion(uint32_t* par, uint32_t a, uint32_t b) {
 __device__ inline uint32_t gload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline uint32_t gfind(const uint32_t* GP, uint32_t a) {
+__device__ inline uint32_t gfind(uint32_t* GP, uint32_t a) {
   uint32_t p = gload(GP + a);
   while (p != a) {
-    a = p;
+    const uint32_t gp = gload(GP + p);
+    if (gp == p) return p;
+    atomicMin(GP + a, gp);  // path halving, as in lfind_h
+    a = gp;
     p = gload(GP + a);
   }
   return a;
@@ -306,5 +352,78 @@ struct Nbhd {
     return m;
   }
 };
+
+
+// Bit sets over the backward neighbours of connectivity NB.
+template <int NB>
+struct NbSets {
+  using NH = Nbhd<NB>;
+  static constexpr uint32_t ALL = (1u << NH::N) - 1u;
+  __host__ __device__ static constexpr uint32_t sel(int axis, int val) {
+    uint32_t m = 0;
+    for (int i = 0; i < NH::N; i++) {
+      const Off o = NH::off(i);
+      const int c = axis == 0 ? o.dx : axis == 1 ? o.dy : o.dz;
+      if (c == val) m |= 1u << i;
+    }
+    return m;
+  }
+  // neighbours after neighbour j in raster order (z, then y, then x)
+  __host__ __device__ static constexpr uint32_t later(int j) {
+    uint32_t m = 0;
+    const Off a = NH::off(j);
+    for (int i = 0; i < NH::N; i++) {
+      const Off b = NH::off(i);
+      const bool gt = b.dz != a.dz ? b.dz > a.dz : (b.dy != a.dy ? b.dy > a.dy : b.dx > a.dx);
+      if (gt) m |= 1u << i;
+    }
+    return m;
+  }
+  static constexpr uint32_t XLO = sel(0, -1), XHI = sel(0, 1), X0 = sel(0, 0);
+  static constexpr uint32_t YLO = sel(1, -1), YHI = sel(1, 1), Y0 = sel(1, 0);
+  static constexpr uint32_t ZLO = sel(2, -1), Z0 = sel(2, 0);
+};
+
+// Where a tile voxel's backward neighbours fall: H = neighbours in the halo;
+// SX0/SX2 (SY0/SY2, SZ0) = neighbours in the neighbouring tile below/above
+// along x (y, z).
+struct Border {
+  uint32_t H, SX0, SX2, SY0, SY2, SZ0;
+};
+
+template <int NB>
+__device__ inline Border make_border(bool xlo, bool xhi, bool ylo, bool yhi, bool zlo) {
+  using S = NbSets<NB>;
+  Border b;
+  b.SX0 = xlo ? S::XLO : 0u;
+  b.SX2 = xhi ? S::XHI : 0u;
+  b.SY0 = ylo ? S::YLO : 0u;
+  b.SY2 = yhi ? S::YHI : 0u;
+  b.SZ0 = zlo ? S::ZLO : 0u;
+  b.H = b.SX0 | b.SX2 | b.SY0 | b.SY2 | b.SZ0;
+  return b;
+}
+
+// Neighbours that a linked neighbour j may cover: adjacent to j, and the pair
+// (j, i) is unioned inside this tile's structure -- its later voxel lies in
+// the tile, or both are halo voxels of the same neighbouring tile.
+template <int NB>
+__device__ inline uint32_t cover_from(int j, const Border& b) {
+  using NH = Nbhd<NB>;
+  using S = NbSets<NB>;
+  const uint32_t m = NH::adj(j);
+  if (!b.H) return m;
+  const uint32_t lat = S::later(j);
+  uint32_t proc = lat & ~b.H;
+  if (!((b.H >> j) & 1u)) {
+    proc |= S::ALL & ~lat;
+  } else {
+    const uint32_t sx = ((b.SX0 >> j) & 1u) ? b.SX0 : ((b.SX2 >> j) & 1u) ? b.SX2 : (S::ALL & ~(b.SX0 | b.SX2));
+    const uint32_t sy = ((b.SY0 >> j) & 1u) ? b.SY0 : ((b.SY2 >> j) & 1u) ? b.SY2 : (S::ALL & ~(b.SY0 | b.SY2));
+    const uint32_t sz = ((b.SZ0 >> j) & 1u) ? b.SZ0 : (S::ALL & ~b.SZ0);
+    proc |= sx & sy & sz & b.H;
+  }
+  return m & proc;
+}
 
 }  // namespace cc3d_amd

@@ -337,10 +337,10 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     __syncthreads();
   }
 
-  // 3. tile voxels
+  // 3. tile voxels (not unrolled: keeps the kernel inside the instruction cache)
   unsigned long long my_epl = 0, my_prov = 0;
   long long my_first = 0x7FFFFFFFFFFFFFFFll, my_last = -1;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < PT; k++) {
     const int ps = t + WG * k;
     const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
