@@ -83,9 +83,9 @@ template <class G>
 __device__ inline uint32_t lroot_key(const GlobalParams& p, int64_t gx, int64_t gy, int64_t gz) {
   const int64_t tx = gx / G::TX, ty = gy / G::TY, tz = G::IS3D ? gz / G::TZ : 0;
   const int64_t tile = tx + p.ntx * (ty + p.nty * tz);
-  const int pos = (int)(gx - tx * G::TX) + G::TX * ((int)(gy - ty * G::TY) + G::TY * (int)(gz - tz * G::TZ));
+  const int ji = G::jidx((int)(gx - tx * G::TX), (int)(gy - ty * G::TY), (int)(gz - tz * G::TZ));
   const uint32_t R = p.TR[tile];
-  const uint32_t j = R <= 255 ? p.J8[tile * G::TV + pos] : p.J16[tile * G::TV + pos];
+  const uint32_t j = R <= 255 ? p.J8[tile * G::TV + ji] : p.J16[tile * G::TV + ji];
   if (j == 0) return 0xFFFFFFFFu;
   return p.RL[tile * G::TV + j - 1];
 }
@@ -328,25 +328,42 @@ __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
   }
 }
 
-// Stage 2: every voxel of the tile gets its local root's label.
+// Stage 2: every voxel of the tile gets its local root's label.  Thread t
+// owns the column the local kernel gave it: its 8 root indices are one 8- or
+// 16-byte load.
 template <class OUT, class G>
-__global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* out) {
+__global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
+  static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
   const int t = threadIdx.x;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   const int64_t tz = tile / (p.ntx * p.nty);
   const int64_t tr = tile - tz * p.ntx * p.nty;
   const int64_t ty = tr / p.ntx, tx = tr - ty * p.ntx;
   const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
-  const bool narrow = p.TR[tile] <= 255;
-  const uint32_t* LT = p.RL + tile * G::TV;
+  const uint32_t* __restrict__ LT = p.RL + tile * G::TV;
+  uint32_t j[8];
+  if (p.TR[tile] <= 255) {
+    const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + tile * G::TV)[t];
 #pragma unroll
-  for (int k = 0; k < G::PER_THREAD; k++) {
-    const int ps = t + WG * k;
-    const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-    const int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    if (gx >= p.sx || gy >= p.sy || gz >= p.sz) continue;
-    const uint32_t j = narrow ? p.J8[tile * G::TV + ps] : p.J16[tile * G::TV + ps];
-    out[gx + p.sx * (gy + p.sy * gz)] = j ? (OUT)LT[j - 1] : (OUT)0;
+    for (int k = 0; k < 8; k++) j[k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
+  } else {
+    const uint4 w = reinterpret_cast<const uint4*>(p.J16 + tile * G::TV)[t];
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+  }
+  uint32_t lab[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) lab[k] = j[k] ? LT[j[k] - 1] : 0u;
+  int lx, ly, lz;
+  G::voxel(t, 0, lx, ly, lz);
+  const int64_t gx = x0 + lx;
+  if (gx >= p.sx) return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    G::voxel(t, k, lx, ly, lz);
+    const int64_t gy = y0 + ly, gz = z0 + lz;
+    if (gy < p.sy && gz < p.sz) out[gx + p.sx * (gy + p.sy * gz)] = (OUT)lab[k];
   }
 }
 

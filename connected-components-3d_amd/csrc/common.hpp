@@ -246,19 +246,20 @@ __device__ inline uint32_t lfind(uint32_t* par, uint32_t a) {
 }
 
 // Lock-free union: link the larger root under the smaller with atomicMin and
-// retry from the displaced parent when another lane got there first.
-__device__ inline void lunion(uint32_t* par, uint32_t a, uint32_t b) {
+// retry from the displaced parent when another lane got there first.  Returns
+// a node of the merged set (a good starting point for the next union of a).
+__device__ inline uint32_t lunion(uint32_t* par, uint32_t a, uint32_t b) {
   while (true) {
     a = lfind_h(par, a);
     b = lfind_h(par, b);
-    if (a == b) return;
+    if (a == b) return a;
     if (a < b) {
       const uint32_t t = a;
       a = b;
       b = t;
     }
     const uint32_t old = atomicMin(&par[a], b);
-    if (old == a) return;
+    if (old == a) return b;
     a = old;
   }
 }
@@ -354,26 +355,29 @@ struct Nbhd {
 };
 
 
-// Bit sets over the backward neighbours of connectivity NB.
-template <int NB>
+// Bit sets over the positions of the full backward neighbourhood (the 13 of
+// 26-connectivity in 3D, the 4 of 8-connectivity in 2D); the smaller
+// neighbourhoods are prefixes of these tables.
+template <bool IS3D>
 struct NbSets {
-  using NH = Nbhd<NB>;
-  static constexpr uint32_t ALL = (1u << NH::N) - 1u;
+  static constexpr int N = IS3D ? 13 : 4;
+  static constexpr uint32_t ALL = (1u << N) - 1u;
+  __host__ __device__ static constexpr Off off(int i) { return IS3D ? off3(i) : off2(i); }
   __host__ __device__ static constexpr uint32_t sel(int axis, int val) {
     uint32_t m = 0;
-    for (int i = 0; i < NH::N; i++) {
-      const Off o = NH::off(i);
+    for (int i = 0; i < N; i++) {
+      const Off o = off(i);
       const int c = axis == 0 ? o.dx : axis == 1 ? o.dy : o.dz;
       if (c == val) m |= 1u << i;
     }
     return m;
   }
-  // neighbours after neighbour j in raster order (z, then y, then x)
+  // positions after position j in raster order (z, then y, then x)
   __host__ __device__ static constexpr uint32_t later(int j) {
     uint32_t m = 0;
-    const Off a = NH::off(j);
-    for (int i = 0; i < NH::N; i++) {
-      const Off b = NH::off(i);
+    const Off a = off(j);
+    for (int i = 0; i < N; i++) {
+      const Off b = off(i);
       const bool gt = b.dz != a.dz ? b.dz > a.dz : (b.dy != a.dy ? b.dy > a.dy : b.dx > a.dx);
       if (gt) m |= 1u << i;
     }
@@ -384,16 +388,17 @@ struct NbSets {
   static constexpr uint32_t ZLO = sel(2, -1), Z0 = sel(2, 0);
 };
 
-// Where a tile voxel's backward neighbours fall: H = neighbours in the halo;
-// SX0/SX2 (SY0/SY2, SZ0) = neighbours in the neighbouring tile below/above
-// along x (y, z).
+// Where a tile voxel's backward neighbourhood leaves the tile.  H = positions
+// in the halo; SX0/SX2 (SY0/SY2, SZ0) = positions in the neighbouring tile
+// below/above along x (y, z).  All zero for a voxel whose neighbourhood lies
+// inside the tile.
 struct Border {
   uint32_t H, SX0, SX2, SY0, SY2, SZ0;
 };
 
-template <int NB>
+template <bool IS3D>
 __device__ inline Border make_border(bool xlo, bool xhi, bool ylo, bool yhi, bool zlo) {
-  using S = NbSets<NB>;
+  using S = NbSets<IS3D>;
   Border b;
   b.SX0 = xlo ? S::XLO : 0u;
   b.SX2 = xhi ? S::XHI : 0u;
@@ -404,26 +409,31 @@ __device__ inline Border make_border(bool xlo, bool xhi, bool ylo, bool yhi, boo
   return b;
 }
 
-// Neighbours that a linked neighbour j may cover: adjacent to j, and the pair
-// (j, i) is unioned inside this tile's structure -- its later voxel lies in
-// the tile, or both are halo voxels of the same neighbouring tile.
-template <int NB>
-__device__ inline uint32_t cover_from(int j, const Border& b) {
-  using NH = Nbhd<NB>;
-  using S = NbSets<NB>;
-  const uint32_t m = NH::adj(j);
-  if (!b.H) return m;
+// Positions i such that the pair (j, i) is unioned inside this tile's
+// structure: its later voxel lies in the tile, or both voxels are halo voxels
+// of the same neighbouring tile.
+template <bool IS3D>
+__device__ inline uint32_t proc_mask(int j, const Border& b) {
+  using S = NbSets<IS3D>;
+  if (!b.H) return S::ALL;
   const uint32_t lat = S::later(j);
-  uint32_t proc = lat & ~b.H;
+  uint32_t m = lat & ~b.H;
   if (!((b.H >> j) & 1u)) {
-    proc |= S::ALL & ~lat;
+    m |= S::ALL & ~lat;
   } else {
     const uint32_t sx = ((b.SX0 >> j) & 1u) ? b.SX0 : ((b.SX2 >> j) & 1u) ? b.SX2 : (S::ALL & ~(b.SX0 | b.SX2));
     const uint32_t sy = ((b.SY0 >> j) & 1u) ? b.SY0 : ((b.SY2 >> j) & 1u) ? b.SY2 : (S::ALL & ~(b.SY0 | b.SY2));
     const uint32_t sz = ((b.SZ0 >> j) & 1u) ? b.SZ0 : (S::ALL & ~b.SZ0);
-    proc |= sx & sy & sz & b.H;
+    m |= sx & sy & sz & b.H;
   }
-  return m & proc;
+  return m;
+}
+
+// Neighbours that a linked neighbour j may cover (connectivity NB): adjacent
+// to j, with the pair (j, i) unioned inside this tile's structure.
+template <int NB>
+__device__ inline uint32_t cover_from(int j, const Border& b) {
+  return Nbhd<NB>::adj(j) & proc_mask<Nbhd<NB>::IS3D>(j, b);
 }
 
 }  // namespace cc3d_amd

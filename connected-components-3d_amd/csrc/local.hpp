@@ -18,18 +18,30 @@
 //               lie within delta of the global min/max, else all matches
 //               (cc3d/cc3d_continuous.hpp:335-384)
 //
+// Work layout.  Thread t owns one column of PER_THREAD voxels along the
+// column axis (z in 3D, y in 2D); the column neighbour is backward neighbour 0
+// (E in 3D, K in 2D).  A voxel linked to its column predecessor inside the
+// tile continues a run: its parent is set to the run's first voxel with a
+// plain store before any union, so most voxels of large components never
+// touch the union-find.  The voxels that still need neighbour unions (run
+// starts, and run members whose column link does not cover the rest of the
+// neighbourhood) are compacted into an LDS queue and processed densely.
+//
 // Covering: a linked backward neighbour need not be unioned when it is
 // adjacent to another neighbour already connected to p and linked to it,
 // because that pair's own union (made when the later of the two was
 // processed) connects them.  For multilabel/binary links are transitive, so
 // adjacency suffices; for continuous links it takes an exactly equal value or
-// a direct match with the first neighbour.  6/4-connectivity uses the
-// phantom voxels J, D, B (2D: J) of the 2x2(x2) cube.
+// a direct match with neighbour 0.  6/4-connectivity uses the phantom voxels
+// D, B, J (2D: J) of the 2x2(x2) cube.  On a tile border a pair may only be
+// relied on when it is unioned inside this tile's structure (bit masks in
+// common.hpp), so tiles never lean on each other's cross-tile links.
 //
 // Outputs per tile:
-//   RL[tile*TV + r]  global key of local root r (roots in ballot order)
+//   RL[tile*TV + r]  global key of local root r
 //   TR[tile]         number of local roots R
-//   J8/J16           per voxel: 0 (background) or 1 + index of its local root;
+//   J8/J16           per voxel (thread-major: thread t's column is entries
+//                    8t..8t+7): 0 (background) or 1 + index of its local root;
 //                    8-bit when R <= 255, else 16-bit
 //   GP[key]          initialised to key for every local root
 //   REQ/TQ           cross-tile union requests: (root position, halo index)
@@ -60,221 +72,212 @@ struct LocalParams {
   Counters* ctr;
 };
 
+// Offsets of backward position i in the staged box and in the tile's node
+// numbering (raster order).
 template <class G>
-__device__ inline int tile_slot(int bx, int by, int bz) {
-  const int sxl = bx == 0 ? 0 : (bx <= G::TX ? 1 : 2);
-  const int syl = by == 0 ? 0 : (by <= G::TY ? 1 : 2);
-  const int szl = G::IS3D ? (bz == 0 ? 0 : 1) : 0;
-  return sxl + 3 * (syl + 3 * szl);
+__host__ __device__ constexpr int box_off(int i) {
+  const Off o = NbSets<G::IS3D>::off(i);
+  return o.dx + G::BX * (o.dy + G::BY * o.dz);
 }
-
 template <class G>
-__device__ inline bool in_box(int bx, int by, int bz) {
-  return bx >= 0 && bx < G::BX && by >= 0 && by < G::BY && bz >= 0 && bz < G::BZ;
+__host__ __device__ constexpr int node_off(int i) {
+  const Off o = NbSets<G::IS3D>::off(i);
+  return o.dx + G::TX * (o.dy + G::TY * o.dz);
 }
 
-// Raster order of two box positions (z, then y, then x).
-__device__ inline bool box_later(int ax, int ay, int az, int bx, int by, int bz) {
-  if (az != bz) return az > bz;
-  if (ay != by) return ay > by;
-  return ax > bx;
+// Continuous 2D-8 shortcut (cc3d/cc3d_continuous.hpp:341-349): both values
+// within delta of the global minimum and maximum.
+template <class T>
+__device__ inline bool q3_span(T cur, T vb, T delta, T gmin, T gmax) {
+  const T lo = (vb < cur) ? vb : cur;
+  const T hi = (cur < vb) ? vb : cur;
+  const T d1 = lo - gmin, d2 = gmax - hi;
+  return d1 <= delta && d2 <= delta;
 }
 
-// Is the adjacent pair (a, b) unioned inside this tile's structure: by the
-// tile phase (its later voxel is a tile voxel) or by the halo phase (both are
-// halo voxels of the same neighbouring tile)?
-template <class G>
-__device__ inline bool pair_here(int ax, int ay, int az, int bx, int by, int bz) {
-  const bool ta = G::is_tile(ax, ay, az), tb = G::is_tile(bx, by, bz);
-  if (box_later(ax, ay, az, bx, by, bz) ? ta : tb) return true;
-  return !ta && !tb && tile_slot<G>(ax, ay, az) == tile_slot<G>(bx, by, bz);
+// Link between a foreground voxel and its column predecessor.
+template <class T, int MODE, int NB, int QK>
+__device__ inline bool column_link(T cur, T prev, T delta, T gmin, T gmax) {
+  if constexpr (QK == Q_BIN4) {
+    return fg(prev);  // rows after the first link up whenever the pixel above is set
+  } else if constexpr (QK == Q_CONT8) {
+    return prev == cur || (fg(prev) && (q3_span(cur, prev, delta, gmin, gmax) || match(cur, prev, delta)));
+  } else if constexpr (MODE == M_CONT) {
+    return fg(prev) && (match(cur, prev, delta) || (NB == 26 && prev == cur));
+  } else if constexpr (MODE == M_MULTI) {
+    return prev == cur;
+  } else {
+    return fg(prev);
+  }
 }
 
-// All backward neighbours of this tile voxel lie inside the tile.
-template <class G, int NB>
-__device__ inline bool interior_voxel(int bx, int by, int bz) {
-  if constexpr (NB == 26 || NB == 18)
-    return bx >= 2 && bx <= G::TX - 1 && by >= 2 && by <= G::TY - 1 && bz >= 2;
-  else if constexpr (NB == 6)
-    return bx >= 2 && by >= 2 && bz >= 2;
-  else if constexpr (NB == 8)
-    return bx >= 2 && bx <= G::TX - 1 && by >= 2;
-  else
-    return bx >= 2 && by >= 2;
-}
-
-
-// Unions for one foreground voxel at box (bx,by,bz).  HALO: restrict to halo
-// neighbours in the same neighbouring tile.  Returns the number of linked
-// neighbours found (0 <=> the reference gives this voxel a new label).
+// Unions for one foreground voxel p (box index b, node pn, box coordinates
+// bx, by, bz).  HALO: a halo voxel, restricted to the positions in `elig`
+// (halo voxels of the same neighbouring tile).  COL: p continues a run, so
+// its link to neighbour 0 is already in place.  Returns the number of linked
+// backward neighbours (0 <=> the reference gives p a new provisional label).
 template <class T, int MODE, int NB, int QK, class G, bool HALO>
-__device__ inline int link_voxel(const T* sv, uint32_t* par, int bx, int by, int bz, T cur,
-                                 T delta, T gmin, T gmax, int64_t gy, int slot) {
+__device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn, int bx, int by, int bz,
+                                 T cur, const Border& bd, uint32_t elig, bool col, T delta, T gmin, T gmax,
+                                 bool row0) {
   using NH = Nbhd<NB>;
-  const uint32_t pn = G::template node_box<QK>(bx, by, bz);
-  auto ok = [&](int qx, int qy, int qz) -> bool {
-    if (!HALO) return true;
-    return in_box<G>(qx, qy, qz) && !G::is_tile(qx, qy, qz) && tile_slot<G>(qx, qy, qz) == slot;
+  constexpr bool IS3D = G::IS3D;
+  auto val = [&](int i) -> T { return sv[b + box_off<G>(i)]; };
+  auto nd = [&](int i) -> uint32_t {
+    const Off o = NbSets<IS3D>::off(i);
+    if (HALO) return G::TV + G::halo_index(bx + o.dx, by + o.dy, bz + o.dz);
+    if (QK != Q_BLOCK8 && !bd.H) return pn + node_off<G>(i);
+    return G::template node_box<QK>(bx + o.dx, by + o.dy, bz + o.dz);
   };
-  auto val = [&](int qx, int qy, int qz) -> T { return sv[G::box(qx, qy, qz)]; };
-  auto nd = [&](int qx, int qy, int qz) -> uint32_t {
-    return G::template node_box<QK>(qx, qy, qz);
-  };
+  auto cover = [&](int i) -> uint32_t { return HALO ? NH::adj(i) : cover_from<NB>(i, bd); };
+  auto ok = [&](int i) -> bool { return !HALO || ((elig >> i) & 1u); };
+  uint32_t a = pn;
 
   if constexpr (QK == Q_BIN4) {
-    // cc3d/cc3d_binary.hpp:822-935
-    if (gy == 0) {
-      if (val(bx - 1, by, bz) == cur) {
-        lunion(par, pn, nd(bx - 1, by, bz));
+    // positions K M J L = 0 1 2 3; cc3d/cc3d_binary.hpp:822-935
+    if (row0) {
+      if (val(1) == cur) {
+        lunion(par, a, nd(1));
         return 1;
       }
       return 0;
     }
-    const bool C = fg(val(bx, by - 1, bz));
-    const bool B = fg(val(bx - 1, by, bz));
-    const bool D = fg(val(bx - 1, by - 1, bz));
+    const bool C = col || fg(val(0));
+    const bool B = fg(val(1));
+    const bool D = fg(val(2));
     int n = 0;
     if (C) {
-      lunion(par, pn, nd(bx, by - 1, bz));
+      if (!col) a = lunion(par, a, nd(0));
       n++;
     }
     if (B && !(C && D)) {
-      lunion(par, pn, nd(bx - 1, by, bz));
+      lunion(par, a, nd(1));
       n++;
     }
     return n;
   } else if constexpr (QK == Q_CONT8) {
     // cc3d/cc3d_continuous.hpp:335-384
-    const T vb = val(bx, by - 1, bz);
-    if (vb == cur) {
-      lunion(par, pn, nd(bx, by - 1, bz));
-      return 1;
-    }
-    if (fg(vb)) {
-      const T lo = (vb < cur) ? vb : cur;
-      const T hi = (cur < vb) ? vb : cur;
-      const T d1 = lo - gmin, d2 = gmax - hi;
-      if (d1 <= delta && d2 <= delta) {
-        lunion(par, pn, nd(bx, by - 1, bz));
+    int n = 0;
+    if (col) {
+      n = 1;  // a match-only link to K: the other matches follow
+    } else {
+      const T vb = val(0);
+      if (vb == cur || (fg(vb) && q3_span(cur, vb, delta, gmin, gmax))) {
+        lunion(par, a, nd(0));
         return 1;
       }
     }
-    int n = 0;
-    const int qxs[4] = {bx, bx - 1, bx + 1, bx - 1};
-    const int qys[4] = {by - 1, by - 1, by - 1, by};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const T q = val(qxs[i], qys[i], bz);
+      if (i == 0 && col) continue;
+      const T q = val(i);
       if (fg(q) && match(cur, q, delta)) {
-        lunion(par, pn, nd(qxs[i], qys[i], bz));
+        a = lunion(par, a, nd(i));
         n++;
       }
     }
     return n;
   } else if constexpr (MODE != M_CONT && (NB == 6 || NB == 4)) {
-    // face connectivity: M, K (and E) with phantoms J, D, B
-    auto lk = [&](int qx, int qy, int qz) -> bool {
-      if (!ok(qx, qy, qz)) return false;
-      const T q = val(qx, qy, qz);
+    auto lk = [&](int i) -> bool {
+      if (!ok(i)) return false;
+      const T q = val(i);
       return MODE == M_MULTI ? (q == cur) : fg(q);
     };
-    // a phantom path a-c-b connects a and b when both of its pairs are
-    // processed in this tile's structure
-    auto path = [&](int ax, int ay, int az, int cx, int cy, int cz, int qx, int qy, int qz) -> bool {
-      return HALO || (pair_here<G>(ax, ay, az, cx, cy, cz) && pair_here<G>(cx, cy, cz, qx, qy, qz));
+    // a phantom path x - c - y connects x and y when both pairs are unioned here
+    auto path = [&](int x, int c, int y) -> bool {
+      if (HALO) return true;
+      return ((proc_mask<IS3D>(x, bd) >> c) & 1u) && ((proc_mask<IS3D>(c, bd) >> y) & 1u);
     };
-    const bool lM = lk(bx - 1, by, bz);
-    const bool lK = lk(bx, by - 1, bz);
-    int n = 0;
-    if (lM) {
-      lunion(par, pn, nd(bx - 1, by, bz));
-      n++;
-    }
-    if (lK) {
-      if (!(lM && lk(bx - 1, by - 1, bz) && path(bx - 1, by, bz, bx - 1, by - 1, bz, bx, by - 1, bz)))
-        lunion(par, pn, nd(bx, by - 1, bz));
-      n++;
-    }
     if constexpr (NB == 6) {
-      if (lk(bx, by, bz - 1)) {
-        const bool viaD = lM && lk(bx - 1, by, bz - 1) && path(bx - 1, by, bz, bx - 1, by, bz - 1, bx, by, bz - 1);
-        const bool viaB = lK && lk(bx, by - 1, bz - 1) && path(bx, by - 1, bz, bx, by - 1, bz - 1, bx, by, bz - 1);
-        if (!(viaD || viaB)) lunion(par, pn, nd(bx, by, bz - 1));
-        n++;
-      }
+      // positions E K M B D J = 0 1 2 3 5 7
+      const bool lE = col || lk(0);
+      if (lE && !col) a = lunion(par, a, nd(0));
+      const bool lM = lk(2);
+      if (lM && !(lE && lk(5) && path(0, 5, 2))) a = lunion(par, a, nd(2));
+      const bool lK = lk(1);
+      if (lK && !((lE && lk(3) && path(0, 3, 1)) || (lM && lk(7) && path(2, 7, 1)))) lunion(par, a, nd(1));
+      return (int)lE + (int)lM + (int)lK;
+    } else {
+      // positions K M J = 0 1 2
+      const bool lK = col || lk(0);
+      if (lK && !col) a = lunion(par, a, nd(0));
+      const bool lM = lk(1);
+      if (lM && !(lK && lk(2) && path(0, 2, 1))) lunion(par, a, nd(1));
+      return (int)lK + (int)lM;
+    }
+  } else if constexpr (MODE != M_CONT) {
+    // 26 / 18 / 8: transitive links, adjacency covering
+    uint32_t cov = col ? (cover(0) | 1u) : 0u;
+    int n = col ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < NH::N; i++) {
+      if (!ok(i) || ((cov >> i) & 1u)) continue;
+      const T q = val(i);
+      const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
+      if (!l) continue;
+      n++;
+      a = lunion(par, a, nd(i));
+      cov |= cover(i);
     }
     return n;
   } else {
-    // Covering masks.  Interior voxels (and the halo phase, where every pair is
-    // two halo voxels of one neighbouring tile) use plain adjacency.  On a tile
-    // border a neighbour j may cover i only when the pair (j, i) is itself
-    // unioned in this tile's structure: the later voxel of the pair lies in the
-    // tile, or both are halo voxels of the same neighbouring tile.  Otherwise
-    // the tiles' local forests could lean on each other's cross-tile links.
-    const bool interior = HALO || interior_voxel<G, NB>(bx, by, bz);
-    auto cover_from = [&](int j) -> uint32_t {
-      uint32_t m = NH::adj(j);
-      if (interior) return m;
-      const Off oj = NH::off(j);
-      uint32_t r = 0;
-#pragma unroll
-      for (int i = 0; i < NH::N; i++) {
-        if (!((m >> i) & 1u)) continue;
-        const Off oi = NH::off(i);
-        if (pair_here<G>(bx + oj.dx, by + oj.dy, bz + oj.dz, bx + oi.dx, by + oi.dy, bz + oi.dz)) r |= 1u << i;
-      }
-      return r;
-    };
-    if constexpr (MODE != M_CONT) {
-      // 26 / 18 / 8: transitive links
-      uint32_t cov = 0;
-      int n = 0;
-#pragma unroll
-      for (int i = 0; i < NH::N; i++) {
-        if ((cov >> i) & 1u) continue;
-        const Off o = NH::off(i);
-        const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
-        if (!ok(qx, qy, qz)) continue;
-        const T q = val(qx, qy, qz);
-        const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
-        if (!l) continue;
-        n++;
-        lunion(par, pn, nd(qx, qy, qz));
-        cov |= cover_from(i);
-      }
-      return n;
-    } else {
-      // continuous: exact-equality covering plus direct matches with neighbour 0
-      uint32_t cov = 0;
-      int n = 0;
-      bool hub = false;
-      uint32_t hubcov = 0;
-      T hv = T(0);
-#pragma unroll
-      for (int i = 0; i < NH::N; i++) {
-        if ((cov >> i) & 1u) continue;
-        const Off o = NH::off(i);
-        const int qx = bx + o.dx, qy = by + o.dy, qz = bz + o.dz;
-        if (!ok(qx, qy, qz)) continue;
-        const T q = val(qx, qy, qz);
-        bool l = fg(q) && match(cur, q, delta);
-        if (NB == 26 && i == 0 && q == cur) l = true;
-        if (!l) continue;
-        n++;
-        bool covered = false;
-        if (i > 0 && hub && ((hubcov >> i) & 1u) && match(hv, q, delta)) covered = true;
-        if (!covered) lunion(par, pn, nd(qx, qy, qz));
-        const uint32_t cf = cover_from(i);
-        if (i == 0) {
-          hub = true;
-          hv = q;
-          hubcov = cf;
-        }
-        if (q == cur) cov |= cf;
-      }
-      return n;
+    // continuous: exact-equality covering plus direct matches with neighbour 0
+    uint32_t cov = 0;
+    int n = 0;
+    bool hub = false;
+    uint32_t hubcov = 0;
+    T hv = T(0);
+    if (col) {
+      hv = val(0);
+      hub = true;
+      hubcov = cover(0);
+      n = 1;
+      cov = 1u | (hv == cur ? hubcov : 0u);
     }
+#pragma unroll
+    for (int i = 0; i < NH::N; i++) {
+      if (!ok(i) || ((cov >> i) & 1u)) continue;
+      const T q = val(i);
+      bool l = fg(q) && match(cur, q, delta);
+      if (NB == 26 && i == 0 && q == cur) l = true;
+      if (!l) continue;
+      n++;
+      const bool covered = i > 0 && hub && ((hubcov >> i) & 1u) && match(hv, q, delta);
+      if (!covered) a = lunion(par, a, nd(i));
+      const uint32_t cf = cover(i);
+      if (i == 0) {
+        hub = true;
+        hv = q;
+        hubcov = cf;
+      }
+      if (q == cur) cov |= cf;
+    }
+    return n;
   }
+}
+
+// Positions a halo voxel at box (bx, by, bz) may link to: backward neighbours
+// inside the box that are halo voxels of the same neighbouring tile.
+template <class G>
+__device__ inline uint32_t halo_elig(int bx, int by, int bz) {
+  using S = NbSets<G::IS3D>;
+  const bool xmid = bx >= 1 && bx <= G::TX;
+  const bool ymid = by >= 1 && by <= G::TY;
+  const uint32_t ex = xmid ? (S::X0 | (bx >= 2 ? S::XLO : 0u) | (bx <= G::TX - 1 ? S::XHI : 0u)) : S::X0;
+  const uint32_t ey = ymid ? (S::Y0 | (by >= 2 ? S::YLO : 0u) | (by <= G::TY - 1 ? S::YHI : 0u)) : S::Y0;
+  uint32_t ez = S::ALL;
+  if (G::IS3D) ez = bz == 0 ? S::Z0 : (S::Z0 | (bz >= 2 ? S::ZLO : 0u));
+  return ex & ey & ez;
+}
+
+// Is a foreground run member fully linked by its column link alone?
+template <class T, int MODE, int NB, int QK>
+__device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, T gmin, T gmax) {
+  if constexpr (QK == Q_CONT8) return prev == cur || q3_span(cur, prev, delta, gmin, gmax);
+  else if constexpr (QK == Q_BIN4) return false;
+  else if constexpr (NB == 6 || NB == 4) return false;  // face links: M and K still checked
+  else if constexpr (MODE == M_CONT) return col_covers_all && prev == cur;
+  else return col_covers_all;
 }
 
 template <class T, int MODE, int NB, int QK, class G>
@@ -283,12 +286,15 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
   constexpr int PT = G::PER_THREAD;
   constexpr int HPT = (G::HALO + WG - 1) / WG;
   constexpr bool PIECES = !(QK == Q_BIN4 || QK == Q_CONT8);
+  constexpr bool IS3D = G::IS3D;
+  constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
+  static_assert(HPT <= 32, "halo piece flags fit one word");
 
   __shared__ T sv[G::BOXN];
   __shared__ uint32_t par[G::NN];
-  __shared__ uint8_t hpiece[G::HALO];
+  __shared__ uint16_t queue[G::TV];
   __shared__ uint32_t wbase[PT * NW + 1];
-  __shared__ uint32_t nreq;
+  __shared__ uint32_t qcount, nreq;
   __shared__ unsigned long long red_epl[NW], red_prov[NW];
   __shared__ long long red_first[NW], red_last[NW];
 
@@ -308,18 +314,49 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     gmax = from_bits<T>(p.minmax[1]);
   }
 
-  // 1. stage tile + halo; voxels outside the volume read as background
-  for (int b = t; b < G::BOXN; b += WG) {
-    const int bz = b / (G::BX * G::BY);
-    const int r = b - bz * (G::BX * G::BY);
-    const int by = r / G::BX, bx = r - by * G::BX;
-    const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
-    T v = T(0);
-    if (gx >= 0 && gx < sx && gy >= 0 && gy < sy && gz >= 0 && gz < sz) v = in[gx + sx * (gy + sy * gz)];
-    sv[b] = v;
+  // this thread's column
+  int lx, ly0, lz0;
+  G::voxel(t, 0, lx, ly0, lz0);
+  const int64_t gx = x0 + lx;
+  const int b0 = G::box(lx + 1, ly0 + 1, IS3D ? 1 : 0);
+
+  // 1. stage the column (registers + LDS) and the halo; voxels outside the
+  //    volume read as background.  Column runs get their parent here.
+  T col[PT];
+  uint32_t runs = 0;  // bit k: voxel k continues the run of voxel k-1
+  {
+    uint32_t start = 0;
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      const int64_t gy = y0 + vy, gz = z0 + vz;
+      T v = T(0);
+      if (gx < sx && gy < sy && gz < sz) v = in[gx + sx * (gy + sy * gz)];
+      col[k] = v;
+      sv[b0 + k * CSTEP] = v;
+      const uint32_t nk = (uint32_t)G::template node<QK>(vx, vy, vz);
+      if (k > 0 && fg(v) && column_link<T, MODE, NB, QK>(v, col[k - 1], delta, gmin, gmax)) {
+        runs |= 1u << k;
+      } else {
+        start = nk;
+      }
+      par[nk] = start;
+    }
   }
-  for (int n = t; n < G::NN; n += WG) par[n] = n;
-  if (t == 0) nreq = 0;
+  for (int h = t; h < G::HALO; h += WG) {
+    int bx, by, bz;
+    G::halo_coords(h, bx, by, bz);
+    const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+    T v = T(0);
+    if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
+    sv[G::box(bx, by, bz)] = v;
+    par[G::TV + h] = G::TV + h;
+  }
+  if (t == 0) {
+    qcount = 0;
+    nreq = 0;
+  }
   __syncthreads();
 
   // 2. halo pieces: links between halo voxels of the same neighbouring tile
@@ -327,67 +364,122 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     for (int h = t; h < G::HALO; h += WG) {
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
-      const T cur = sv[G::box(bx, by, bz)];
+      const int b = G::box(bx, by, bz);
+      const T cur = sv[b];
       if (!fg(cur)) continue;
-      link_voxel<T, MODE, NB, QK, G, true>(sv, par, bx, by, bz, cur, delta, gmin, gmax, 0,
-                                           tile_slot<G>(bx, by, bz));
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, G::TV + h, bx, by, bz, cur, Border{}, halo_elig<G>(bx, by, bz),
+                                           false, delta, gmin, gmax, false);
     }
-    __syncthreads();
-    for (int h = t; h < G::HALO; h += WG) hpiece[h] = (par[G::TV + h] == (uint32_t)(G::TV + h));
-    __syncthreads();
   }
 
-  // 3. tile voxels (not unrolled: keeps the kernel inside the instruction cache)
+  // 3. counters over the column and the work queue: voxels still needing
+  //    neighbour unions (run starts first, run members from the back)
   unsigned long long my_epl = 0, my_prov = 0;
   long long my_first = 0x7FFFFFFFFFFFFFFFll, my_last = -1;
+  uint32_t needA = 0, needB = 0;
+  {
+    const bool xlo = lx == 0, xhi = lx == G::TX - 1;
+    const bool ylo_col = IS3D ? ly0 == 0 : false;  // 2D: only voxel 0 of the first column group
+    const bool yhi_col = IS3D ? ly0 == G::TY - 1 : false;
+    const Border bdc = make_border<IS3D>(xlo, xhi, ylo_col, yhi_col, false);
+    const bool covers_all = (cover_from<NB>(0, bdc) | 1u) == ((1u << Nbhd<NB>::N) - 1u);
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+      const T cur = col[k];
+      if (!fg(cur)) continue;
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      if (cur != sv[b0 + k * CSTEP - 1]) my_epl++;
+      const long long row = (y0 + vy) + sy * (z0 + vz);
+      my_first = row < my_first ? row : my_first;
+      my_last = row > my_last ? row : my_last;
+      if ((runs >> k) & 1u) {
+        if (!column_done<T, MODE, NB, QK>(cur, col[k - 1], covers_all, delta, gmin, gmax)) needB |= 1u << k;
+      } else {
+        needA |= 1u << k;
+      }
+    }
+    // wave-level exclusive scan of the packed counts, one LDS atomic per wave
+    const uint32_t cnt = (uint32_t)__popc(needA) | ((uint32_t)__popc(needB) << 16);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(&qcount, incl);
+    base = __shfl(base, 63);
+    uint32_t ia = (base & 0xFFFFu) + ((incl - cnt) & 0xFFFFu);
+    uint32_t ib = (base >> 16) + ((incl - cnt) >> 16);
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      const uint16_t ps = (uint16_t)G::pos(vx, vy, vz);
+      if ((needA >> k) & 1u) queue[ia++] = ps;
+      if ((needB >> k) & 1u) queue[G::TV - 1 - ib++] = ps;
+    }
+  }
+  __syncthreads();
+
+  // halo pieces are the roots left by step 2 (without pieces every
+  // foreground halo voxel stands for itself)
+  uint32_t hpiece = 0;
+#pragma unroll
+  for (int i = 0; i < HPT; i++) {
+    const int h = t + WG * i;
+    if (h < G::HALO) {
+      int bx, by, bz;
+      G::halo_coords(h, bx, by, bz);
+      if (fg(sv[G::box(bx, by, bz)]) && (!PIECES || par[G::TV + h] == (uint32_t)(G::TV + h))) hpiece |= 1u << i;
+    }
+  }
+  const uint32_t nA = qcount & 0xFFFFu, nQ = nA + (qcount >> 16);
+  __syncthreads();
+
+  // 4. queued voxels
 #pragma unroll 1
-  for (int k = 0; k < PT; k++) {
-    const int ps = t + WG * k;
-    const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-    const int bx = lx + 1, by = ly + 1, bz = G::IS3D ? lz + 1 : 0;
+  for (uint32_t q = t; q < nQ; q += WG) {
+    const bool colq = q >= nA;
+    const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
+    const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
+    const int bx = vx + 1, by = vy + 1, bz = IS3D ? vz + 1 : 0;
     const int b = G::box(bx, by, bz);
     const T cur = sv[b];
-    if (!fg(cur)) continue;
-    const int64_t gy = y0 + ly, gz = z0 + lz;
-    if (cur != sv[b - 1]) my_epl++;
-    const long long row = gy + sy * gz;
-    my_first = row < my_first ? row : my_first;
-    my_last = row > my_last ? row : my_last;
-    const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, bx, by, bz, cur, delta, gmin,
-                                                           gmax, gy, 0);
+    const Border bd = make_border<IS3D>(vx == 0, vx == G::TX - 1, vy == 0, IS3D && vy == G::TY - 1, IS3D && vz == 0);
+    const uint32_t pn = (uint32_t)G::template node<QK>(vx, vy, vz);
+    const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, 0u, colq, delta,
+                                                          gmin, gmax, y0 + vy == 0);
     if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
   }
   __syncthreads();
 
-  // 4. resolve roots
+  // 5. resolve roots: run members share their run start's root
   uint32_t root[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++) {
-    const int ps = t + WG * k;
-    const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-    const T cur = sv[G::box(lx + 1, ly + 1, G::IS3D ? lz + 1 : 0)];
-    root[k] = fg(cur) ? lfind(par, G::template node<QK>(lx, ly, lz)) : 0xFFFFFFFFu;
+    int vx, vy, vz;
+    G::voxel(t, k, vx, vy, vz);
+    if (!fg(col[k])) root[k] = 0xFFFFFFFFu;
+    else if ((runs >> k) & 1u) root[k] = root[k > 0 ? k - 1 : 0];
+    else root[k] = lfind(par, (uint32_t)G::template node<QK>(vx, vy, vz));
   }
   uint32_t hroot[HPT];
 #pragma unroll
   for (int i = 0; i < HPT; i++) {
     const int h = t + WG * i;
-    hroot[i] = 0xFFFFFFFFu;
-    // without pieces every linked halo voxel is its own request
-    if (h < G::HALO && (!PIECES || hpiece[h])) {
-      int bx, by, bz;
-      G::halo_coords(h, bx, by, bz);
-      if (fg(sv[G::box(bx, by, bz)])) hroot[i] = lfind(par, G::TV + h);
-    }
+    hroot[i] = ((hpiece >> i) & 1u) ? lfind(par, G::TV + h) : 0xFFFFFFFFu;
   }
-  // 5. rank local roots (ballot order: slice k, then wave, then lane)
+
+  // 6. rank local roots (ballot order: slice k, then wave, then lane)
   bool isroot[PT];
   uint64_t bal[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++) {
-    const int ps = t + WG * k;
-    const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-    isroot[k] = root[k] == (uint32_t)G::template node<QK>(lx, ly, lz);
+    int vx, vy, vz;
+    G::voxel(t, k, vx, vy, vz);
+    isroot[k] = root[k] == (uint32_t)G::template node<QK>(vx, vy, vz);
     bal[k] = __ballot(isroot[k]);
     if (lane == 0) wbase[k * NW + wv] = __popcll(bal[k]);
   }
@@ -410,40 +502,49 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     if (isroot[k]) {
       const uint32_t rk = wbase[k * NW + wv] + __popcll(bal[k] & ltmask);
       par[root[k]] = rk;
-      const int ps = t + WG * k;
-      const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-      const uint32_t key = gkey<QK>(x0 + lx, y0 + ly, z0 + lz, sx, sy, p.osx);
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      const uint32_t key = gkey<QK>(x0 + vx, y0 + vy, z0 + vz, sx, sy, p.osx);
       RLt[rk] = key;
       p.GP[key] = key;
     }
   }
   __syncthreads();
 
-  // 6. per-voxel root index
-  const bool narrow = R <= 255;
+  // 7. per-voxel root index, thread-major: one 8- or 16-byte store per thread
+  static_assert(PT == 8, "J stores assume 8 voxels per thread");
+  uint32_t jv[PT];
 #pragma unroll
-  for (int k = 0; k < PT; k++) {
-    const int ps = t + WG * k;
-    const uint32_t j = root[k] == 0xFFFFFFFFu ? 0u : par[root[k]] + 1u;
-    if (narrow) p.J8[tile * G::TV + ps] = (uint8_t)j;
-    else p.J16[tile * G::TV + ps] = (uint16_t)j;
+  for (int k = 0; k < PT; k++) jv[k] = root[k] == 0xFFFFFFFFu ? 0u : par[root[k]] + 1u;
+  if (R <= 255) {
+    uint64_t w = 0;
+#pragma unroll
+    for (int k = 0; k < PT; k++) w |= (uint64_t)jv[k] << (8 * k);
+    reinterpret_cast<uint64_t*>(p.J8 + tile * G::TV)[t] = w;
+  } else {
+    uint4 w;
+    w.x = jv[0] | (jv[1] << 16);
+    w.y = jv[2] | (jv[3] << 16);
+    w.z = jv[4] | (jv[5] << 16);
+    w.w = jv[6] | (jv[7] << 16);
+    reinterpret_cast<uint4*>(p.J16 + tile * G::TV)[t] = w;
   }
 
-  // 7. cross-tile requests: one per halo piece attached to a local component
+  // 8. cross-tile requests: one per halo piece attached to a local component
   uint32_t* REQt = p.REQ + tile * G::HALO;
 #pragma unroll
   for (int i = 0; i < HPT; i++) {
     const int h = t + WG * i;
     const uint32_t r = hroot[i];
     if (r < (uint32_t)G::TV) {
-      int lx, ly, lz;
-      G::template node_to_local<QK>(r, lx, ly, lz);
+      int vx, vy, vz;
+      G::template node_to_local<QK>(r, vx, vy, vz);
       const uint32_t slot = atomicAdd(&nreq, 1u);
-      REQt[slot] = ((uint32_t)G::pos(lx, ly, lz) << 16) | (uint32_t)h;
+      REQt[slot] = ((uint32_t)G::pos(vx, vy, vz) << 16) | (uint32_t)h;
     }
   }
 
-  // 8. counters
+  // 9. counters
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     my_epl += __shfl_xor(my_epl, o);
