@@ -50,6 +50,51 @@ __global__ void k_init(Counters* c, uint32_t* bits, int64_t nwords) {
   for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
 }
 
+// Folds the local kernel's per-tile counters into Counters.
+__global__ __launch_bounds__(WG) void k_tile_counters(const TileCnt* tc, int64_t ntiles, Counters* c) {
+  __shared__ unsigned long long se[WG / 64], sp[WG / 64];
+  __shared__ long long sf[WG / 64], sl[WG / 64];
+  unsigned long long e = 0, pv = 0;
+  long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < ntiles; i += (int64_t)gridDim.x * WG) {
+    const TileCnt v = tc[i];
+    e += v.epl;
+    pv += v.prov;
+    f = v.first_row < f ? v.first_row : f;
+    l = v.last_row > l ? v.last_row : l;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o);
+    pv += __shfl_xor(pv, o);
+    const long long f2 = __shfl_xor(f, o), l2 = __shfl_xor(l, o);
+    f = f2 < f ? f2 : f;
+    l = l2 > l ? l2 : l;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    se[wv] = e;
+    sp[wv] = pv;
+    sf[wv] = f;
+    sl[wv] = l;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < WG / 64; w++) {
+      e += se[w];
+      pv += sp[w];
+      f = sf[w] < f ? sf[w] : f;
+      l = sl[w] > l ? sl[w] : l;
+    }
+    if (e) atomicAdd(&c->epl, e);
+    if (pv) atomicAdd(&c->prov, pv);
+    if (l >= 0) {
+      atomicMin(&c->first_row, f);
+      atomicMax(&c->last_row, l);
+    }
+  }
+}
+
 template <class G>
 struct TileIdx {
   int64_t ntx, nty, ntz;
@@ -672,7 +717,7 @@ struct Geometry {
   int TV = 0, HALO = 0;
   int64_t nwords = 0, nblk = 0;
   int64_t qblocks = 0, qwords = 0, qnblk = 0;  // binary 2D-8 block bitmask
-  size_t oGP, oRL, oTR, oTQ, oJ8, oJ16, oREQ, oBITS, oPRE, oBSUM, oCTR, oMM, oQB, oQP, oQS, total;
+  size_t oGP, oRL, oTR, oTQ, oTC, oJ8, oJ16, oREQ, oBITS, oPRE, oBSUM, oCTR, oMM, oQB, oQP, oQS, total;
 };
 
 static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary,
@@ -730,6 +775,7 @@ static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connect
   g.oRL = take(slots * 4);
   g.oTR = take((size_t)g.ntiles * 4);
   g.oTQ = take((size_t)g.ntiles * 4);
+  g.oTC = take((size_t)g.ntiles * sizeof(TileCnt));
   g.oJ8 = take(slots);
   g.oJ16 = take(slots * 2);
   g.oREQ = take((size_t)g.ntiles * g.HALO * 4);
@@ -1017,7 +1063,7 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
   lp.J16 = reinterpret_cast<uint16_t*>(ws + g.oJ16);
   lp.REQ = reinterpret_cast<uint32_t*>(ws + g.oREQ);
   lp.TQ = reinterpret_cast<uint32_t*>(ws + g.oTQ);
-  lp.ctr = gp.ctr;
+  lp.tcnt = reinterpret_cast<TileCnt*>(ws + g.oTC);
   hipError_t e;
   {
   ProfScope ps_("k_local", s);
@@ -1028,6 +1074,15 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
   if (e != hipSuccess) {
     cc3d_amd_job_free(j);
     return fail(CC3D_AMD_ERR_HIP, std::string("k_local: ") + hipGetErrorString(e));
+  }
+  {
+    ProfScope ps_("k_tile_counters", s);
+    hipLaunchKernelGGL(k_tile_counters, dim3(grid_for(g.ntiles, WG * 16, 256)), dim3(WG), 0, s, lp.tcnt, g.ntiles,
+                       gp.ctr);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    cc3d_amd_job_free(j);
+    return fail(CC3D_AMD_ERR_HIP, "k_tile_counters launch failed");
   }
   rc = g.is3d ? run_global<Geo3>(j) : run_global<Geo2>(j);
   if (rc == CC3D_AMD_OK && (j->count_prov || j->stale)) {

@@ -36,6 +36,13 @@ struct Counters {
   unsigned long long pad[2];
 };
 
+// Per-tile partial counters written by the local kernel (plain stores; one
+// reduction kernel folds them into Counters).
+struct TileCnt {
+  uint32_t epl, prov;
+  long long first_row, last_row;
+};
+
 // Tile geometry.  A tile is TX x TY x TZ voxels; the workgroup stages it in LDS
 // together with a one-voxel halo on the sides that backward neighbours reach
 // (x-1..x+1, y-1..y+1 and z-1 in 3D; x-1..x+1 and y-1 in 2D).

@@ -48,8 +48,10 @@
 //                    for each halo "piece" (halo voxels connected through links
 //                    that stay inside one neighbouring tile) touching a local
 //                    component
-//   counters         epl transitions, provisional-label count, first/last
-//                    foreground row (cc3d/cc3d.hpp:287-315)
+//   tcnt[tile]       epl transitions, provisional-label count, first/last
+//                    foreground row (cc3d/cc3d.hpp:287-315), folded by
+//                    k_tile_counters (one store per tile: thousands of
+//                    workgroups atomically updating one address serialise)
 #pragma once
 
 #include "common.hpp"
@@ -69,7 +71,7 @@ struct LocalParams {
   uint16_t* J16;
   uint32_t* REQ;
   uint32_t* TQ;
-  Counters* ctr;
+  TileCnt* tcnt;
 };
 
 // Offsets of backward position i in the staged box and in the tile's node
@@ -114,8 +116,8 @@ __device__ inline bool column_link(T cur, T prev, T delta, T gmin, T gmax) {
 // Unions for one foreground voxel p (box index b, node pn, box coordinates
 // bx, by, bz).  HALO: a halo voxel, restricted to the positions in `elig`
 // (halo voxels of the same neighbouring tile).  COL: p continues a run, so
-// its link to neighbour 0 is already in place.  Returns the number of linked
-// backward neighbours (0 <=> the reference gives p a new provisional label).
+// its link to neighbour 0 is already in place.  Returns 0 exactly when p has
+// no linked backward neighbour (the reference gives p a new provisional label).
 template <class T, int MODE, int NB, int QK, class G, bool HALO>
 __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn, int bx, int by, int bz,
                                  T cur, const Border& bd, uint32_t elig, bool col, T delta, T gmin, T gmax,
@@ -206,22 +208,37 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
       return (int)lK + (int)lM;
     }
   } else if constexpr (MODE != M_CONT) {
-    // 26 / 18 / 8: transitive links, adjacency covering
+    // 26 / 18 / 8: transitive links, adjacency covering.  A covered neighbour
+    // that is linked is connected to p as well, so its cover counts too.
+    // Pass 1 visits the neighbours the column link already covers, pass 2 the
+    // rest; only linked neighbours still uncovered when visited are unioned.
+    constexpr uint32_t ALLN = (1u << NH::N) - 1u;
+    const uint32_t okm = HALO ? (elig & ALLN) : ALLN;
     uint32_t cov = col ? (cover(0) | 1u) : 0u;
-    int n = col ? 1 : 0;
+    uint32_t seen = col ? 1u : 0u;
+    auto lnk = [&](T q) -> bool { return MODE == M_MULTI ? (q == cur) : fg(q); };
+    if (col) {
+#pragma unroll
+      for (int i = 1; i < NH::N; i++) {
+        if (!((okm >> i) & 1u) || !((cov >> i) & 1u)) continue;
+        seen |= 1u << i;
+        if (lnk(val(i))) cov |= cover(i);
+      }
+    }
 #pragma unroll
     for (int i = 0; i < NH::N; i++) {
-      if (!ok(i) || ((cov >> i) & 1u)) continue;
+      if (((cov | seen | ~okm) & ALLN) == ALLN) break;
+      if (!((okm >> i) & 1u) || ((seen >> i) & 1u)) continue;
       const T q = val(i);
-      const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
-      if (!l) continue;
-      n++;
-      a = lunion(par, a, nd(i));
+      if (!lnk(q)) continue;
+      if (!((cov >> i) & 1u)) a = lunion(par, a, nd(i));
       cov |= cover(i);
     }
-    return n;
+    return cov ? 1 : 0;
   } else {
-    // continuous: exact-equality covering plus direct matches with neighbour 0
+    // continuous: exact-equality covering plus direct matches with neighbour
+    // 0.  Covered neighbours are still read: an exactly equal one extends the
+    // cover.
     uint32_t cov = 0;
     int n = 0;
     bool hub = false;
@@ -236,13 +253,13 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
     }
 #pragma unroll
     for (int i = 0; i < NH::N; i++) {
-      if (!ok(i) || ((cov >> i) & 1u)) continue;
+      if (!ok(i) || (col && i == 0)) continue;
       const T q = val(i);
       bool l = fg(q) && match(cur, q, delta);
       if (NB == 26 && i == 0 && q == cur) l = true;
       if (!l) continue;
       n++;
-      const bool covered = i > 0 && hub && ((hubcov >> i) & 1u) && match(hv, q, delta);
+      const bool covered = ((cov >> i) & 1u) || (i > 0 && hub && ((hubcov >> i) & 1u) && match(hv, q, delta));
       if (!covered) a = lunion(par, a, nd(i));
       const uint32_t cf = cover(i);
       if (i == 0) {
@@ -484,14 +501,17 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     if (lane == 0) wbase[k * NW + wv] = __popcll(bal[k]);
   }
   __syncthreads();
-  if (t == 0) {
-    uint32_t s = 0;
-    for (int i = 0; i < PT * NW; i++) {
-      const uint32_t c = wbase[i];
-      wbase[i] = s;
-      s += c;
+  if (wv == 0) {
+    static_assert(PT * NW <= 64, "one wave scans the per-wave root counts");
+    const uint32_t c = lane < PT * NW ? wbase[lane] : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
     }
-    wbase[PT * NW] = s;
+    if (lane < PT * NW) wbase[lane] = incl - c;
+    if (lane == 63) wbase[PT * NW] = incl;
   }
   __syncthreads();
   const uint32_t R = wbase[PT * NW];
@@ -571,12 +591,12 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     }
     p.TR[tile] = R;
     p.TQ[tile] = nreq;
-    if (e) atomicAdd(&p.ctr->epl, e);
-    if (pv) atomicAdd(&p.ctr->prov, pv);
-    if (l >= 0) {
-      atomicMin(&p.ctr->first_row, f);
-      atomicMax(&p.ctr->last_row, l);
-    }
+    TileCnt c;
+    c.epl = (uint32_t)e;
+    c.prov = (uint32_t)pv;
+    c.first_row = f;
+    c.last_row = l;
+    p.tcnt[tile] = c;
   }
 }
 

@@ -70,6 +70,28 @@ def _rand_case(rng, big=False):
   return arr, kw
 
 
+def test_reference_vectors(gpu):
+  """Every committed reference vector (tests/golden/ref_vectors.npz: outputs of the
+  reference compiled from its own sources) through the product, bit for bit."""
+  import os
+  import sys
+  import cc3d
+  sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+  import make_golden
+  cases = make_golden.load()
+  assert len(cases) > 600
+  for i, (arr, kw, out, N, err) in enumerate(cases):
+    print(f"vector {i} shape={arr.shape} dtype={arr.dtype} kw={kw}", flush=True)
+    got = _run(cc3d.connected_components, arr, kw)
+    if err:
+      assert got[0] == "err" and got[1] == err, f"vector {i}: reference raised {err}, product {got[:2]}"
+      continue
+    assert got[0] == "ok", f"vector {i}: product raised {got[1]}: {got[2]}"
+    assert got[2] == N, f"vector {i}: N {got[2]} vs {N}"
+    assert got[1].dtype == out.dtype and got[1].shape == out.shape, f"vector {i}: dtype/shape"
+    assert np.array_equal(got[1], out), f"vector {i}: labels differ"
+
+
 def test_random_small(gpu):
   import cc3d
   rng = np.random.default_rng(12345)
