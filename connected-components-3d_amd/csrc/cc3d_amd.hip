@@ -95,17 +95,6 @@ __global__ __launch_bounds__(WG) void k_tile_counters(const TileCnt* tc, int64_t
   }
 }
 
-template <class G>
-struct TileIdx {
-  int64_t ntx, nty, ntz;
-  __device__ inline void coords(int64_t tile, int64_t& tx, int64_t& ty, int64_t& tz) const {
-    tz = tile / (ntx * nty);
-    const int64_t r = tile - tz * ntx * nty;
-    ty = r / ntx;
-    tx = r - ty * ntx;
-  }
-};
-
 struct GlobalParams {
   int64_t sx, sy, sz, osx;
   int64_t ntx, nty, ntz, ntiles;
@@ -121,18 +110,45 @@ struct GlobalParams {
   uint32_t* bsum;
   int64_t nwords;
   Counters* ctr;
+  // batch of volumes (nvol > 0): per-volume table and first tile of each
+  const BatchVol* vols;
+  const int64_t* vtile0;
+  int nvol;
 };
+
+// Geometry of the volume owning batch tile `slot`.
+struct VolGeo {
+  int64_t sx, sy, sz, osx, ntx, nty, ntz, tile0;
+  uint32_t key0;
+};
+
+__device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
+  VolGeo g;
+  if (p.nvol == 0) {
+    g.sx = p.sx; g.sy = p.sy; g.sz = p.sz; g.osx = p.osx;
+    g.ntx = p.ntx; g.nty = p.nty; g.ntz = p.ntz;
+    g.tile0 = 0;
+    g.key0 = 0;
+  } else {
+    const BatchVol& v = p.vols[find_vol(p.vtile0, p.nvol, slot)];
+    g.sx = v.sx; g.sy = v.sy; g.sz = v.sz; g.osx = v.osx;
+    g.ntx = v.ntx; g.nty = v.nty; g.ntz = v.ntz;
+    g.tile0 = v.tile0;
+    g.key0 = v.key0;
+  }
+  return g;
+}
 
 // Key of the local root of voxel (gx, gy, gz); 0xFFFFFFFF for background.
 template <class G>
-__device__ inline uint32_t lroot_key(const GlobalParams& p, int64_t gx, int64_t gy, int64_t gz) {
+__device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int64_t gx, int64_t gy, int64_t gz) {
   const int64_t tx = gx / G::TX, ty = gy / G::TY, tz = G::IS3D ? gz / G::TZ : 0;
-  const int64_t tile = tx + p.ntx * (ty + p.nty * tz);
+  const int64_t slot = v.tile0 + tx + v.ntx * (ty + v.nty * tz);
   const int ji = G::jidx((int)(gx - tx * G::TX), (int)(gy - ty * G::TY), (int)(gz - tz * G::TZ));
-  const uint32_t R = p.TR[tile];
-  const uint32_t j = R <= 255 ? p.J8[tile * G::TV + ji] : p.J16[tile * G::TV + ji];
+  const uint32_t R = p.TR[slot];
+  const uint32_t j = R <= 255 ? p.J8[slot * G::TV + ji] : p.J16[slot * G::TV + ji];
   if (j == 0) return 0xFFFFFFFFu;
-  return p.RL[tile * G::TV + j - 1];
+  return p.RL[slot * G::TV + j - 1];
 }
 
 // Cross-tile unions: one wave per tile walks that tile's requests.
@@ -141,22 +157,24 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
-  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
-    const uint32_t nq = p.TQ[tile];
+  for (int64_t slot = wave; slot < p.ntiles; slot += nwaves) {
+    const uint32_t nq = p.TQ[slot];
     if (nq == 0) continue;
-    const int64_t tz = tile / (p.ntx * p.nty);
-    const int64_t tr = tile - tz * p.ntx * p.nty;
-    const int64_t ty = tr / p.ntx, tx = tr - ty * p.ntx;
+    const VolGeo v = vol_geo(p, slot);
+    const int64_t tile = slot - v.tile0;
+    const int64_t tz = tile / (v.ntx * v.nty);
+    const int64_t tr = tile - tz * v.ntx * v.nty;
+    const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
     const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
     for (uint32_t i = lane; i < nq; i += 64) {
-      const uint32_t rq = p.REQ[tile * G::HALO + i];
+      const uint32_t rq = p.REQ[slot * G::HALO + i];
       const int ps = rq >> 16, h = rq & 0xFFFF;
       const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-      const uint32_t a = gkey<QK>(x0 + lx, y0 + ly, z0 + lz, p.sx, p.sy, p.osx);
+      const uint32_t a = gkey<QK>(x0 + lx, y0 + ly, z0 + lz, v.sx, v.sy, v.osx) + v.key0;
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
       const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
-      const uint32_t b = lroot_key<G>(p, gx, gy, gz);
+      const uint32_t b = lroot_key<G>(p, v, gx, gy, gz);
       if (b != 0xFFFFFFFFu) gunion(p.GP, a, b);
     }
   }
@@ -263,7 +281,8 @@ __global__ __launch_bounds__(WG) void k_periodic(GlobalParams p, const T* in, in
     if (!fg(va)) continue;
     const bool link = MODE == M_BINARY ? fg(vb) : (va == vb);
     if (!link) continue;
-    const uint32_t ka = lroot_key<G>(p, ax, ay, az), kb = lroot_key<G>(p, bx, by, bz);
+    const VolGeo v = vol_geo(p, 0);
+    const uint32_t ka = lroot_key<G>(p, v, ax, ay, az), kb = lroot_key<G>(p, v, bx, by, bz);
     if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) gunion(p.GP, ka, kb);
   }
 }
@@ -355,20 +374,23 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
   }
 }
 
-// Final label of every local root, written over its RL entry.
+// Final label of every local root, written over its RL entry: 1 + the rank of
+// its global root's key among the roots of its volume.
 template <class G>
 __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
-  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
-    const uint32_t R = p.TR[tile];
+  for (int64_t slot = wave; slot < p.ntiles; slot += nwaves) {
+    const uint32_t R = p.TR[slot];
+    if (R == 0) continue;
+    const uint32_t base = p.nvol ? p.prefix[vol_geo(p, slot).key0 >> 5] : 0u;
     for (uint32_t j = lane; j < R; j += 64) {
-      const uint32_t g = p.RL[tile * G::TV + j];
+      const uint32_t g = p.RL[slot * G::TV + j];
       const uint32_t r = p.GP[g];
       const uint32_t wd = r >> 5;
       const uint32_t below = p.bits[wd] & ((1u << (r & 31)) - 1u);
-      p.RL[tile * G::TV + j] = p.prefix[wd] + __popc(below) + 1u;
+      p.RL[slot * G::TV + j] = p.prefix[wd] - base + __popc(below) + 1u;
     }
   }
 }
@@ -377,22 +399,22 @@ __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
 // owns the column the local kernel gave it: its 8 root indices are one 8- or
 // 16-byte load.
 template <class OUT, class G>
-__global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
+__device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_t tile, const VolGeo& v,
+                                     OUT* __restrict__ out) {
   static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
   const int t = threadIdx.x;
-  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
-  const int64_t tz = tile / (p.ntx * p.nty);
-  const int64_t tr = tile - tz * p.ntx * p.nty;
-  const int64_t ty = tr / p.ntx, tx = tr - ty * p.ntx;
+  const int64_t tz = tile / (v.ntx * v.nty);
+  const int64_t tr = tile - tz * v.ntx * v.nty;
+  const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
   const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
-  const uint32_t* __restrict__ LT = p.RL + tile * G::TV;
+  const uint32_t* __restrict__ LT = p.RL + slot * G::TV;
   uint32_t j[8];
-  if (p.TR[tile] <= 255) {
-    const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + tile * G::TV)[t];
+  if (p.TR[slot] <= 255) {
+    const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + slot * G::TV)[t];
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
   } else {
-    const uint4 w = reinterpret_cast<const uint4*>(p.J16 + tile * G::TV)[t];
+    const uint4 w = reinterpret_cast<const uint4*>(p.J16 + slot * G::TV)[t];
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
@@ -403,13 +425,86 @@ __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict
   int lx, ly, lz;
   G::voxel(t, 0, lx, ly, lz);
   const int64_t gx = x0 + lx;
-  if (gx >= p.sx) return;
+  if (gx >= v.sx) return;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     G::voxel(t, k, lx, ly, lz);
     const int64_t gy = y0 + ly, gz = z0 + lz;
-    if (gy < p.sy && gz < p.sz) out[gx + p.sx * (gy + p.sy * gz)] = (OUT)lab[k];
+    if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[k];
   }
+}
+
+template <class OUT, class G>
+__global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
+  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
+  finalize_tile<OUT, G>(p, tile, tile, vol_geo(p, 0), out);
+}
+
+// Batch: each tile writes its own volume's output in that volume's dtype.
+template <class G>
+__global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
+  const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
+  const BatchVol& bv = p.vols[find_vol(p.vtile0, p.nvol, slot)];
+  const VolGeo v = vol_geo(p, slot);
+  if (bv.out_dtype == DT_U16) finalize_tile<uint16_t, G>(p, slot, slot - v.tile0, v, static_cast<uint16_t*>(bv.out));
+  else if (bv.out_dtype == DT_U64) finalize_tile<uint64_t, G>(p, slot, slot - v.tile0, v, static_cast<uint64_t*>(bv.out));
+  else finalize_tile<uint32_t, G>(p, slot, slot - v.tile0, v, static_cast<uint32_t*>(bv.out));
+}
+
+// Batch: per-volume counters from the per-tile records (one block per volume).
+__global__ __launch_bounds__(WG) void k_tile_counters_batch(const TileCnt* tc, const int64_t* vtile0, Counters* ctrs) {
+  __shared__ unsigned long long se[WG / 64], sp[WG / 64];
+  __shared__ long long sf[WG / 64], sl[WG / 64];
+  const int v = blockIdx.x;
+  unsigned long long e = 0, pv = 0;
+  long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
+  for (int64_t i = vtile0[v] + threadIdx.x; i < vtile0[v + 1]; i += WG) {
+    const TileCnt c = tc[i];
+    e += c.epl;
+    pv += c.prov;
+    f = c.first_row < f ? c.first_row : f;
+    l = c.last_row > l ? c.last_row : l;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o);
+    pv += __shfl_xor(pv, o);
+    const long long f2 = __shfl_xor(f, o), l2 = __shfl_xor(l, o);
+    f = f2 < f ? f2 : f;
+    l = l2 > l ? l2 : l;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    se[wv] = e;
+    sp[wv] = pv;
+    sf[wv] = f;
+    sl[wv] = l;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < WG / 64; w++) {
+      e += se[w];
+      pv += sp[w];
+      f = sf[w] < f ? sf[w] : f;
+      l = sl[w] > l ? sl[w] : l;
+    }
+    Counters& c = ctrs[v];
+    c.epl = e;
+    c.prov = pv;
+    c.first_row = f;
+    c.last_row = l;
+  }
+}
+
+// Batch: N of every volume from the prefix at its first key word.
+__global__ void k_batch_N(const BatchVol* vols, int n, const uint32_t* prefix, int64_t nwords, const Counters* total,
+                          Counters* ctrs) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t w0 = vols[v].key0 >> 5;
+  const int64_t w1 = v + 1 < n ? (int64_t)(vols[v + 1].key0 >> 5) : nwords;
+  const uint64_t hi = w1 < nwords ? prefix[w1] : total->N;
+  ctrs[v].N = hi - prefix[w0];
 }
 
 // estimate_provisional_label_count (cc3d/cc3d.hpp:287-315): one wave per row.

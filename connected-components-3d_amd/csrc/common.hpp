@@ -43,6 +43,29 @@ struct TileCnt {
   long long first_row, last_row;
 };
 
+// One volume of a batch (device-side table).
+struct BatchVol {
+  const void* in;
+  void* out;
+  int64_t sx, sy, sz, osx;
+  int64_t ntx, nty, ntz, ntiles;
+  int64_t tile0;      // first tile of this volume in the batch's tile numbering
+  uint32_t key0;      // first key of this volume in the batch's key space (multiple of 32)
+  uint32_t nkeys;
+  int nb, out_dtype;
+};
+
+// Volume of a batch owning batch tile `slot` (vtile0: first tile of each volume).
+__device__ inline int find_vol(const int64_t* vtile0, int n, int64_t slot) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vtile0[mid] <= slot) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Tile geometry.  A tile is TX x TY x TZ voxels; the workgroup stages it in LDS
 // together with a one-voxel halo on the sides that backward neighbours reach
 // (x-1..x+1, y-1..y+1 and z-1 in 3D; x-1..x+1 and y-1 in 2D).

@@ -297,8 +297,27 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
   else return col_covers_all;
 }
 
+// LDS of one tile (declared once per kernel, so kernels that dispatch to
+// several connectivities at run time allocate it once).
+template <class T, class G>
+struct LocalLds {
+  static constexpr int NW = WG / 64;
+  T sv[G::BOXN];
+  uint32_t par[G::NN];
+  uint16_t queue[G::TV];
+  uint16_t hqueue[G::IS3D ? G::HALO : 1];
+  uint32_t wbase[G::PER_THREAD * NW + 1];
+  uint32_t qcount, nreq, hcount;
+  unsigned long long red_epl[NW], red_prov[NW];
+  long long red_first[NW], red_last[NW];
+};
+
+// Labels tile `tile` of the volume described by p.  Per-tile outputs go to
+// index `slot` of RL/J/REQ/TR/TQ/tcnt; keys are offset by `keybase` (both
+// are `tile` and 0 for a single volume; a batch concatenates volumes).
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(WG) void k_local(LocalParams p) {
+__device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t slot, uint32_t keybase,
+                                  LocalLds<T, G>& L) {
   constexpr int NW = WG / 64;
   constexpr int PT = G::PER_THREAD;
   constexpr int HPT = (G::HALO + WG - 1) / WG;
@@ -306,18 +325,20 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
   constexpr bool IS3D = G::IS3D;
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
   static_assert(HPT <= 32, "halo piece flags fit one word");
+  // 3D transitive 26/18: halo staged as runs (see step 1)
+  constexpr bool HRUNS = PIECES && IS3D && MODE != M_CONT && (NB == 26 || NB == 18);
 
-  __shared__ T sv[G::BOXN];
-  __shared__ uint32_t par[G::NN];
-  __shared__ uint16_t queue[G::TV];
-  __shared__ uint32_t wbase[PT * NW + 1];
-  __shared__ uint32_t qcount, nreq;
-  __shared__ unsigned long long red_epl[NW], red_prov[NW];
-  __shared__ long long red_first[NW], red_last[NW];
+  T* sv = L.sv;
+  uint32_t* par = L.par;
+  uint16_t* queue = L.queue;
+  uint16_t* hqueue = L.hqueue;
+  uint32_t* wbase = L.wbase;
+  uint32_t& qcount = L.qcount;
+  uint32_t& nreq = L.nreq;
+  uint32_t& hcount = L.hcount;
 
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
-  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   const int64_t tz = tile / (p.ntx * p.nty);
   const int64_t trem = tile - tz * p.ntx * p.nty;
   const int64_t ty = trem / p.ntx, tx = trem - ty * p.ntx;
@@ -330,6 +351,13 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     gmin = from_bits<T>(p.minmax[0]);
     gmax = from_bits<T>(p.minmax[1]);
   }
+
+  if (t == 0) {
+    qcount = 0;
+    nreq = 0;
+    hcount = 0;
+  }
+  __syncthreads();
 
   // this thread's column
   int lx, ly0, lz0;
@@ -361,23 +389,82 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
       par[nk] = start;
     }
   }
-  for (int h = t; h < G::HALO; h += WG) {
-    int bx, by, bz;
-    G::halo_coords(h, bx, by, bz);
-    const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-    T v = T(0);
-    if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
-    sv[G::box(bx, by, bz)] = v;
-    par[G::TV + h] = G::TV + h;
-  }
-  if (t == 0) {
-    qcount = 0;
-    nreq = 0;
+  if constexpr (HRUNS) {
+    // 3D halo as columns: ring position r (threads 0..RING-1) along z, bottom
+    // plane column bx (threads RING..RING+BX-1) along y.  A halo voxel linked
+    // to its predecessor inside the same neighbouring tile continues a run;
+    // its link then covers every other position of its piece, so only run
+    // starts are queued for step 2.
+    constexpr int RING = G::RING, HLEN = G::TZ > G::BY ? G::TZ : G::BY;
+    if (t < RING + G::BX) {
+      const bool ring = t < RING;
+      int cbx, cby;
+      if (ring) {
+        if (t < G::BX) {
+          cbx = t;
+          cby = 0;
+        } else if (t < 2 * G::BX) {
+          cbx = t - G::BX;
+          cby = G::BY - 1;
+        } else {
+          cbx = ((t - 2 * G::BX) & 1) ? G::BX - 1 : 0;
+          cby = ((t - 2 * G::BX) >> 1) + 1;
+        }
+      } else {
+        cbx = t - RING;
+        cby = 0;
+      }
+      const int len = ring ? G::TZ : G::BY;
+      const int cpos = ring ? 0 : 1;  // E or K
+      T prev = T(0);
+      uint32_t start = 0;
+#pragma unroll
+      for (int k = 0; k < HLEN; k++) {
+        if (k >= len) break;
+        const int bx = cbx, by = ring ? cby : k, bz = ring ? k + 1 : 0;
+        const int h = ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k;
+        const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = z0 + bz - 1;
+        T v = T(0);
+        if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
+        sv[G::box(bx, by, bz)] = v;
+        const bool linkable = ring ? k >= 1 : (k >= 2 && k <= G::TY);
+        bool member = false;
+        if (fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax)) {
+          const uint32_t el = halo_elig<G>(bx, by, bz);
+          member = ((Nbhd<NB>::adj(cpos) | (1u << cpos)) & el) == (el & ((1u << Nbhd<NB>::N) - 1u));
+        }
+        if (!member) start = G::TV + h;
+        par[G::TV + h] = start;
+        if (fg(v) && !member) hqueue[atomicAdd(&hcount, 1u)] = (uint16_t)h;
+        prev = v;
+      }
+    }
+  } else {
+    for (int h = t; h < G::HALO; h += WG) {
+      int bx, by, bz;
+      G::halo_coords(h, bx, by, bz);
+      const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+      T v = T(0);
+      if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
+      sv[G::box(bx, by, bz)] = v;
+      par[G::TV + h] = G::TV + h;
+    }
   }
   __syncthreads();
 
   // 2. halo pieces: links between halo voxels of the same neighbouring tile
-  if constexpr (PIECES) {
+  if constexpr (HRUNS) {
+    const uint32_t nh = hcount;
+#pragma unroll 1
+    for (uint32_t q = t; q < nh; q += WG) {
+      const int h = hqueue[q];
+      int bx, by, bz;
+      G::halo_coords(h, bx, by, bz);
+      const int b = G::box(bx, by, bz);
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, G::TV + h, bx, by, bz, sv[b], Border{},
+                                           halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
+    }
+  } else if constexpr (PIECES) {
     for (int h = t; h < G::HALO; h += WG) {
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
@@ -516,7 +603,7 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
   __syncthreads();
   const uint32_t R = wbase[PT * NW];
   const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t* RLt = p.RL + tile * G::TV;
+  uint32_t* RLt = p.RL + slot * G::TV;
 #pragma unroll
   for (int k = 0; k < PT; k++) {
     if (isroot[k]) {
@@ -524,7 +611,7 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
       par[root[k]] = rk;
       int vx, vy, vz;
       G::voxel(t, k, vx, vy, vz);
-      const uint32_t key = gkey<QK>(x0 + vx, y0 + vy, z0 + vz, sx, sy, p.osx);
+      const uint32_t key = gkey<QK>(x0 + vx, y0 + vy, z0 + vz, sx, sy, p.osx) + keybase;
       RLt[rk] = key;
       p.GP[key] = key;
     }
@@ -540,18 +627,18 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     uint64_t w = 0;
 #pragma unroll
     for (int k = 0; k < PT; k++) w |= (uint64_t)jv[k] << (8 * k);
-    reinterpret_cast<uint64_t*>(p.J8 + tile * G::TV)[t] = w;
+    reinterpret_cast<uint64_t*>(p.J8 + slot * G::TV)[t] = w;
   } else {
     uint4 w;
     w.x = jv[0] | (jv[1] << 16);
     w.y = jv[2] | (jv[3] << 16);
     w.z = jv[4] | (jv[5] << 16);
     w.w = jv[6] | (jv[7] << 16);
-    reinterpret_cast<uint4*>(p.J16 + tile * G::TV)[t] = w;
+    reinterpret_cast<uint4*>(p.J16 + slot * G::TV)[t] = w;
   }
 
   // 8. cross-tile requests: one per halo piece attached to a local component
-  uint32_t* REQt = p.REQ + tile * G::HALO;
+  uint32_t* REQt = p.REQ + slot * G::HALO;
 #pragma unroll
   for (int i = 0; i < HPT; i++) {
     const int h = t + WG * i;
@@ -574,36 +661,88 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
     my_last = l > my_last ? l : my_last;
   }
   if (lane == 0) {
-    red_epl[wv] = my_epl;
-    red_prov[wv] = my_prov;
-    red_first[wv] = my_first;
-    red_last[wv] = my_last;
+    L.red_epl[wv] = my_epl;
+    L.red_prov[wv] = my_prov;
+    L.red_first[wv] = my_first;
+    L.red_last[wv] = my_last;
   }
   __syncthreads();
   if (t == 0) {
     unsigned long long e = 0, pv = 0;
     long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
     for (int w = 0; w < NW; w++) {
-      e += red_epl[w];
-      pv += red_prov[w];
-      f = red_first[w] < f ? red_first[w] : f;
-      l = red_last[w] > l ? red_last[w] : l;
+      e += L.red_epl[w];
+      pv += L.red_prov[w];
+      f = L.red_first[w] < f ? L.red_first[w] : f;
+      l = L.red_last[w] > l ? L.red_last[w] : l;
     }
-    p.TR[tile] = R;
-    p.TQ[tile] = nreq;
+    p.TR[slot] = R;
+    p.TQ[slot] = nreq;
     TileCnt c;
     c.epl = (uint32_t)e;
     c.prov = (uint32_t)pv;
     c.first_row = f;
     c.last_row = l;
-    p.tcnt[tile] = c;
+    p.tcnt[slot] = c;
   }
+}
+
+template <class T, int MODE, int NB, int QK, class G>
+__global__ __launch_bounds__(WG) void k_local(LocalParams p) {
+  __shared__ LocalLds<T, G> L;
+  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
+  local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
+}
+
+// A batch of 3D volumes in one launch: grid = all tiles of all volumes; each
+// workgroup finds its volume and runs that volume's connectivity.  p carries
+// the shared per-tile arrays; its geometry fields are taken from the table.
+template <class T, int MODE, class G>
+__global__ __launch_bounds__(WG) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
+                                                    int nvol) {
+  __shared__ LocalLds<T, G> L;
+  const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
+  const BatchVol& v = vols[find_vol(vtile0, nvol, slot)];
+  LocalParams q = p;
+  q.in = v.in;
+  q.sx = v.sx;
+  q.sy = v.sy;
+  q.sz = v.sz;
+  q.osx = v.osx;
+  q.ntx = v.ntx;
+  q.nty = v.nty;
+  q.ntz = v.ntz;
+  q.ntiles = v.ntiles;
+  const int64_t tile = slot - v.tile0;
+  if (v.nb == 26) local_tile<T, MODE, 26, Q_NONE, G>(q, tile, slot, v.key0, L);
+  else if (v.nb == 18) local_tile<T, MODE, 18, Q_NONE, G>(q, tile, slot, v.key0, L);
+  else local_tile<T, MODE, 6, Q_NONE, G>(q, tile, slot, v.key0, L);
 }
 
 // Launchers (one translation unit per mode keeps compile times short).
 hipError_t launch_local_multi(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
 hipError_t launch_local_binary(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
 hipError_t launch_local_cont(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
+hipError_t launch_local_batch_multi(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
+                                    int nvol, hipStream_t s);
+hipError_t launch_local_batch_binary(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
+                                     int nvol, hipStream_t s);
+
+template <int MODE>
+hipError_t launch_local_batch_mode(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
+                                   int nvol, hipStream_t s) {
+  const dim3 grid((unsigned)p.ntiles), block(WG);
+  switch (dt) {
+    case DT_U8: hipLaunchKernelGGL((k_local_batch<uint8_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_U16: hipLaunchKernelGGL((k_local_batch<uint16_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_U32: hipLaunchKernelGGL((k_local_batch<uint32_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_U64: hipLaunchKernelGGL((k_local_batch<uint64_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_F32: hipLaunchKernelGGL((k_local_batch<float, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_F64: hipLaunchKernelGGL((k_local_batch<double, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 template <int MODE>
 hipError_t launch_local_mode(int dt, int nb, int qk, const LocalParams& p, hipStream_t s) {

@@ -168,3 +168,48 @@ def test_device_tensor_in_out(gpu):
   out2 = cc3d.connected_components(tc, connectivity=18)
   exp2 = oracle.connected_components(np.ascontiguousarray(vol), connectivity=18)
   assert np.array_equal(out2.cpu().numpy(), exp2)
+
+
+@pytest.mark.parametrize("connectivity", [6, 18, 26])
+def test_single_row_special_case(gpu, connectivity):
+  """automated_test.py:453-483 with a fixed row: one foreground row is labelled
+  as runs of equal values, out dtype uint16."""
+  import cc3d
+  sx, sy, sz = 256, 257, 252
+  y, z = 131, 77
+  img = np.zeros((sx, sy, sz), dtype=np.uint8, order="F")
+  img[:, y, z] = 6
+  out = cc3d.connected_components(img, connectivity=connectivity)
+  epl, start, end = cc3d.estimate_provisional_labels(img)
+  assert (epl, start, end) == (1, y + sy * z, y + sy * z)
+  assert out.dtype == np.uint16
+  gt = np.zeros(img.shape, dtype=np.uint8, order="F")
+  gt[:, y, z] = 1
+  assert np.array_equal(out, gt)
+  img[:100, y, z] = 3
+  gt[100:, y, z] = 2
+  epl, start, end = cc3d.estimate_provisional_labels(img)
+  out = cc3d.connected_components(img, connectivity=connectivity)
+  assert epl == 2 and start == end
+  assert np.array_equal(out, gt)
+
+
+def test_max_labels_is_ignored(gpu):
+  """automated_test.py:490-497: max_labels does not change the result."""
+  import cc3d
+  img = np.arange(0, 64 ** 3).astype(np.uint64).reshape((64, 64, 64))
+  real = cc3d.connected_components(img, max_labels=64 * 64 * 64)
+  assert np.array_equal(real, cc3d.connected_components(img, max_labels=0))
+  assert np.array_equal(real, cc3d.connected_components(img, max_labels=-50))
+  assert np.array_equal(real.ravel(), np.arange(64 ** 3))
+
+
+def test_512_cube_all_distinct(gpu):
+  """automated_test.py:486-488 (every voxel its own component) with the result
+  checked: voxel i of a C-order arange gets label i."""
+  import cc3d
+  img = np.arange(0, 512 ** 3, dtype=np.uint64).reshape((512, 512, 512))
+  out, N = cc3d.connected_components(img, return_N=True)
+  assert N == 512 ** 3 - 1
+  assert out.dtype == np.uint32
+  assert np.array_equal(out.ravel(), np.arange(512 ** 3, dtype=np.uint32))
