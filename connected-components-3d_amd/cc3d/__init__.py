@@ -5,6 +5,7 @@ Drop-in for the labelling path of connected-components-3d 4.0.0
 signatures and results, computed by hand-written HIP kernels for gfx950
 (libcc3d_amd.so, C ABI in include/cc3d_amd.h).
 """
+from ._batch import LabelBatch, connected_components_batch
 from ._core import DimensionError, connected_components, estimate_provisional_labels
 from ._follow import dust, statistics
 
@@ -16,4 +17,6 @@ __all__ = [
   "statistics",
   "dust",
   "DimensionError",
+  "connected_components_batch",
+  "LabelBatch",
 ]

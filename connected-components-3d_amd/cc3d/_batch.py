@@ -1,0 +1,224 @@
+"""Batched labelling: many volumes through one set of GPU launches.
+
+No counterpart in the reference, which labels one volume per call.  Result i
+of `connected_components_batch(volumes, ...)` is exactly
+`connected_components(volumes[i], ...)`: the same labels, N, output dtype and
+exceptions.  Volumes the batched kernels cover (multilabel or binary, 3D
+connectivities or a single plane, no periodic boundary, no delta) share one
+launch of every kernel (cc3d_amd_batch_* in include/cc3d_amd.h); the rest go
+through the single-volume path.
+
+`LabelBatch` keeps a planned batch of device-resident volumes with fixed
+output buffers, for repeated labelling; it can capture the whole launch
+sequence in a HIP graph and replay it.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Any, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from ._core import (_OUT_CODE, _decide, _deliver, _label_one, _prepare, _require_gpu, _stream, _torch,
+                    _torch_out_dtype)
+
+
+def _batchable(pl) -> bool:
+  if pl.flags or pl.periodic_boundary or pl.out_file is not None:
+    return False
+  if pl.connectivity in (26, 18, 6):
+    return not (pl.binary_image and pl.sz == 1)
+  if pl.connectivity in (8, 4):
+    return pl.sz == 1 and not pl.binary_image
+  return False
+
+
+def _volume_table(plans, bufs):
+  vt = (L.Volume * len(plans))()
+  for k, (pl, buf) in enumerate(zip(plans, bufs)):
+    vt[k].inp = buf.data_ptr()
+    vt[k].sx, vt[k].sy, vt[k].sz = pl.sx, pl.sy, pl.sz
+    vt[k].connectivity = pl.connectivity
+  return vt
+
+
+def _label_group(items, results, errors):
+  """Labels plans sharing input dtype and binary mode in one batch."""
+  torch = _torch()
+  plans = [pl for _, pl in items]
+  bufs = [pl.vol.device_buffer(pl.order) for pl in plans]
+  n = len(plans)
+  binary = int(plans[0].binary_image)
+  vt = _volume_table(plans, bufs)
+  lib = L.lib()
+  ws_bytes = int(lib.cc3d_amd_batch_workspace_bytes(vt, n, binary))
+  if ws_bytes == 0:
+    L.check(L.ERR_INVALID)
+  ws = torch.empty(ws_bytes, dtype=torch.uint8, device=bufs[0].device)
+  b = ctypes.c_void_p(0)
+  L.check(lib.cc3d_amd_batch_create(ctypes.byref(b), vt, n, plans[0].code, binary, ctypes.c_void_p(ws.data_ptr()),
+                                    ws_bytes, _stream()))
+  try:
+    L.check(lib.cc3d_amd_batch_begin(b))
+    st = (L.Stats * n)()
+    L.check(lib.cc3d_amd_batch_stats(b, st))
+    odts, Ns = [], []
+    for k, (i, pl) in enumerate(items):
+      try:
+        odt, N = _decide(pl, st[k])
+      except Exception as e:  # the reference raises for this volume
+        errors[i] = e
+        odt, N = np.dtype(np.uint32), 0
+      odts.append(odt)
+      Ns.append(N)
+    outs = [torch.empty(pl.voxels, dtype=_torch_out_dtype(odt), device=bufs[0].device)
+            for pl, odt in zip(plans, odts)]
+    ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+    codes = (ctypes.c_int * n)(*[_OUT_CODE[o] for o in odts])
+    L.check(lib.cc3d_amd_batch_set_outputs(b, ptrs, codes))
+    L.check(lib.cc3d_amd_batch_finish(b))
+    for k, (i, pl) in enumerate(items):
+      if i in errors:
+        continue
+      results[i] = _deliver(outs[k], pl.vol, pl.sx, pl.sy, pl.sz, pl.dims, pl.order, odts[k], None, Ns[k],
+                            pl.return_N)
+  finally:
+    lib.cc3d_amd_batch_free(b)
+
+
+def connected_components_batch(
+  volumes: Sequence[Any],
+  connectivity=26,
+  return_N: bool = False,
+  out_dtype: Optional[Any] = None,
+  binary_image: bool = False,
+  max_labels: int = -1,
+  delta=0,
+  periodic_boundary: bool = False,
+) -> List[Any]:
+  """connected_components over a list of volumes; `connectivity` may be one
+  value or one per volume.  Returns the list of results (each `labels` or
+  `(labels, N)`).  When volumes would raise, the exception of the first such
+  volume is raised after the batch has run."""
+  n = len(volumes)
+  conns = list(connectivity) if isinstance(connectivity, (list, tuple)) else [connectivity] * n
+  if len(conns) != n:
+    raise ValueError("connectivity must be one value or one per volume")
+  results: List[Any] = [None] * n
+  errors = {}
+  plans = []
+  for i, v in enumerate(volumes):
+    try:
+      kind, pl = _prepare(v, max_labels, conns[i], delta, out_dtype, None, periodic_boundary, binary_image,
+                          return_N)
+    except Exception as e:
+      errors[i] = e
+      continue
+    if kind == "done":
+      results[i] = pl
+    else:
+      plans.append((i, pl))
+  if plans:
+    _require_gpu()
+  groups = {}
+  for i, pl in plans:
+    if _batchable(pl):
+      groups.setdefault((pl.code, pl.binary_image), []).append((i, pl))
+    else:
+      try:
+        results[i] = _label_one(pl)
+      except Exception as e:
+        errors[i] = e
+  for items in groups.values():
+    _label_group(items, results, errors)
+  if errors:
+    raise errors[min(errors)]
+  return results
+
+
+class LabelBatch:
+  """A planned batch of device-resident volumes with fixed output buffers.
+
+  volumes: device torch tensors (any layout connected_components accepts).
+  All must be batchable and share dtype and binary mode; out_dtype is fixed
+  (uint16/uint32/uint64) so the whole sequence can run as one HIP graph.
+  `run()` labels through the two-stage launches; `capture()` records begin +
+  finish in a graph and `replay()` relaunches it.  Both return
+  [(labels, N)], raising like connected_components when a volume's estimate
+  does not fit out_dtype or its provisional labels overflow.
+  """
+
+  def __init__(self, volumes, connectivity=26, out_dtype=np.uint32, binary_image=False, stream=None):
+    torch = _torch()
+    _require_gpu()
+    n = len(volumes)
+    conns = list(connectivity) if isinstance(connectivity, (list, tuple)) else [connectivity] * n
+    self.plans = []
+    for v, c in zip(volumes, conns):
+      kind, pl = _prepare(v, -1, c, 0, out_dtype, None, False, binary_image, True)
+      if kind != "plan" or not _batchable(pl):
+        raise ValueError("LabelBatch volumes must be non-empty and batchable")
+      self.plans.append(pl)
+    if len({(pl.code, pl.binary_image) for pl in self.plans}) != 1:
+      raise ValueError("LabelBatch volumes must share dtype and binary mode")
+    self.odt = np.dtype(out_dtype)
+    self.stream = stream if stream is not None else torch.cuda.Stream()
+    self.bufs = [pl.vol.device_buffer(pl.order) for pl in self.plans]
+    dev = self.bufs[0].device
+    self.n = n
+    vt = _volume_table(self.plans, self.bufs)
+    lib = L.lib()
+    ws_bytes = int(lib.cc3d_amd_batch_workspace_bytes(vt, n, int(binary_image)))
+    if ws_bytes == 0:
+      L.check(L.ERR_INVALID)
+    self.ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    self.outs = [torch.empty(pl.voxels, dtype=_torch_out_dtype(self.odt), device=dev) for pl in self.plans]
+    self.b = ctypes.c_void_p(0)
+    torch.cuda.current_stream().synchronize()  # inputs and buffers are ready before the batch stream uses them
+    s = ctypes.c_void_p(self.stream.cuda_stream)
+    L.check(lib.cc3d_amd_batch_create(ctypes.byref(self.b), vt, n, self.plans[0].code, int(binary_image),
+                                      ctypes.c_void_p(self.ws.data_ptr()), ws_bytes, s))
+    ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in self.outs])
+    codes = (ctypes.c_int * n)(*([_OUT_CODE[self.odt]] * n))
+    L.check(lib.cc3d_amd_batch_set_outputs(self.b, ptrs, codes))
+    self.captured = False
+
+  def _results(self, deliver=True):
+    st = (L.Stats * self.n)()
+    L.check(L.lib().cc3d_amd_batch_stats(self.b, st))
+    res = []
+    for k, pl in enumerate(self.plans):
+      odt, N = _decide(pl, st[k])
+      if deliver:
+        res.append(_deliver(self.outs[k], pl.vol, pl.sx, pl.sy, pl.sz, pl.dims, pl.order, odt, None, N, True))
+      else:
+        res.append(N)
+    return res
+
+  def run(self, deliver=True):
+    lib = L.lib()
+    L.check(lib.cc3d_amd_batch_begin(self.b))
+    L.check(lib.cc3d_amd_batch_finish(self.b))
+    return self._results(deliver)
+
+  def capture(self):
+    L.check(L.lib().cc3d_amd_batch_graph_capture(self.b))
+    self.captured = True
+
+  def replay(self, deliver=True):
+    if not self.captured:
+      self.capture()
+    L.check(L.lib().cc3d_amd_batch_graph_launch(self.b))
+    return self._results(deliver)
+
+  def close(self):
+    if self.b:
+      L.lib().cc3d_amd_batch_free(self.b)
+      self.b = ctypes.c_void_p(0)
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:
+      pass

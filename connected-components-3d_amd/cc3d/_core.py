@@ -190,25 +190,17 @@ def estimate_provisional_labels(data) -> Tuple[int, int, int]:
   return (int(epl.value), int(f.value), int(l.value))
 
 
-def connected_components(
-  data,
-  max_labels: int = -1,
-  connectivity: int = 26,
-  return_N: bool = False,
-  delta: Union[int, float] = 0,
-  out_dtype: Optional[Any] = None,
-  out_file=None,
-  periodic_boundary: bool = False,
-  binary_image: bool = False,
-):
-  """Connected components applied to 1D, 2D or 3D images on the GPU.
+class _Plan:
+  """Everything the binding decides about one volume before labelling it
+  (cc3d/fastcc3d.pyx:307-453)."""
+  pass
 
-  Same arguments and results as cc3d.connected_components
-  (cc3d/fastcc3d.pyx:244-626): labels numbered 1..N in order of first
-  appearance in memory (x fastest), output dtype uint16/uint32/uint64 chosen
-  from the provisional-label estimate unless given.  `max_labels` is accepted
-  and ignored, as in the reference.
-  """
+
+def _prepare(data, max_labels, connectivity, delta, out_dtype, out_file, periodic_boundary, binary_image,
+             return_N):
+  """Validation, dims, dtype views and binary detection of one volume.
+
+  Returns ("done", result) for an empty volume, else ("plan", _Plan)."""
   vol = _Vol(data)
   dims = len(vol.shape)
   if dims not in (1, 2, 3):
@@ -229,7 +221,7 @@ def connected_components(
       out = _torch().from_numpy(out)
       if vol.device_input:
         out = out.to(vol.torch_device)
-    return (out, 0) if return_N else out
+    return "done", ((out, 0) if return_N else out)
 
   order = "F" if vol.f_contig else "C"
   shape = list(vol.shape)
@@ -267,42 +259,28 @@ def connected_components(
         f"np.uint16, np.uint32, np.uint64"
       )
 
-  view_dtype = np.dtype(_UNSIGNED_VIEW[np_dtype.itemsize]) if np_dtype.kind != "f" else np_dtype
-  code = _NP_TO_CODE[np_dtype]
-  ufv = _even_ceil(padded_shape[0]) * _even_ceil(padded_shape[1]) * _even_ceil(padded_shape[2])
-
-  def binary_bound(ml: int) -> int:
-    if connectivity in (4, 6):
-      return min(ml, (ufv // 2) + 1)
-    # the reference tests `connectivity == (8,18)`, which never holds
-    return min(ml, (ufv // 8) + 1)
-
-  def choose_out_dtype(max_labels: int):
-    if out_dtype is not None:
-      if np.iinfo(out_dtype).max < max_labels:
-        raise ValueError(
-          f"Explicitly defined out_dtype ({out_dtype}) is too small "
-          f"to contain the estimated maximum number of labels ({max_labels})."
-        )
-      return out_dtype
-    if max_labels < np.iinfo(np.uint16).max:
-      return np.dtype(np.uint16)
-    if max_labels < np.iinfo(np.uint32).max:
-      return np.dtype(np.uint32)
-    return np.dtype(np.uint64)
+  pl = _Plan()
+  pl.data, pl.vol, pl.dims, pl.order = data, vol, dims, order
+  pl.sx, pl.sy, pl.sz, pl.voxels = sx, sy, sz, voxels
+  pl.connectivity, pl.delta, pl.out_dtype, pl.out_file = connectivity, delta, out_dtype, out_file
+  pl.periodic_boundary, pl.return_N = bool(periodic_boundary), return_N
+  pl.binary_image, pl.binary_flag_given = binary_image, binary_flag_given
+  pl.view_dtype = np.dtype(_UNSIGNED_VIEW[np_dtype.itemsize]) if np_dtype.kind != "f" else np_dtype
+  pl.code = _NP_TO_CODE[np_dtype]
+  pl.ufv = _even_ceil(padded_shape[0]) * _even_ceil(padded_shape[1]) * _even_ceil(padded_shape[2])
 
   try:
-    delta_b = _delta_bytes(view_dtype, delta)
+    pl.delta_b = _delta_bytes(pl.view_dtype, delta)
   except OverflowError:
     # The reference converts delta to T only when it calls the C++ core, after
     # choosing the output dtype: report a too-small out_dtype first.
     if binary_flag_given:
-      ml = binary_bound(voxels)
+      ml = _binary_bound(pl, voxels)
     else:
       ml = min(estimate_provisional_labels(data)[0], voxels)
       if binary_image:
-        ml = binary_bound(ml)
-    choose_out_dtype(ml)
+        ml = _binary_bound(pl, ml)
+    _choose_out_dtype(pl, ml)
     raise
 
   flags = 0
@@ -311,53 +289,112 @@ def connected_components(
   # Binary 2D-8 provisional labels (one per new 2x2 block) can only reach the
   # union-find length through an explicit uint16 output; count them then.
   nblocks = ((sx + 1) >> 1) * ((sy + 1) >> 1)
-  q1_count = (binary_image and connectivity == 8 and out_dtype is not None
-              and out_dtype == np.uint16 and nblocks >= 65535)
-  if q1_count:
+  pl.q1_count = (binary_image and connectivity == 8 and out_dtype is not None
+                 and out_dtype == np.uint16 and nblocks >= 65535)
+  if pl.q1_count:
     flags |= L.FLAG_COUNT_PROVISIONAL
+  pl.flags = flags
+  return "plan", pl
 
+
+def _binary_bound(pl, ml: int) -> int:
+  if pl.connectivity in (4, 6):
+    return min(ml, (pl.ufv // 2) + 1)
+  # the reference tests `connectivity == (8,18)`, which never holds
+  return min(ml, (pl.ufv // 8) + 1)
+
+
+def _choose_out_dtype(pl, max_labels: int):
+  if pl.out_dtype is not None:
+    if np.iinfo(pl.out_dtype).max < max_labels:
+      raise ValueError(
+        f"Explicitly defined out_dtype ({pl.out_dtype}) is too small "
+        f"to contain the estimated maximum number of labels ({max_labels})."
+      )
+    return pl.out_dtype
+  if max_labels < np.iinfo(np.uint16).max:
+    return np.dtype(np.uint16)
+  if max_labels < np.iinfo(np.uint32).max:
+    return np.dtype(np.uint32)
+  return np.dtype(np.uint64)
+
+
+def _decide(pl, st):
+  """Out dtype, N and the reference's errors from the stage-1 counters
+  (cc3d/fastcc3d.pyx:381-434, cc3d/cc3d.hpp:107-111)."""
+  if pl.binary_flag_given:
+    epl, ffr, lfr = pl.voxels, 0, pl.sy
+  else:
+    epl, ffr, lfr = int(st.epl), int(st.first_foreground_row), int(st.last_foreground_row)
+  max_labels = min(epl, pl.voxels)
+  if pl.binary_image:
+    max_labels = _binary_bound(pl, max_labels)
+  odt = _choose_out_dtype(pl, max_labels)
+
+  special_row = (pl.delta == 0 and ffr == lfr and ffr >= 0)
+  N = int(st.N)
+  if not special_row:
+    uf_len = int(L.lib().cc3d_amd_union_find_length(
+      max_labels, pl.voxels, pl.connectivity, int(pl.binary_image), _OUT_CODE[odt]))
+    if uf_len == 0:
+      N = 0  # the reference returns before labelling (max_labels == 0)
+    elif (not (pl.binary_image and pl.connectivity == 8) or pl.q1_count) and int(st.provisional) >= uf_len:
+      raise RuntimeError("maximum length exception")
+  return odt, N
+
+
+def connected_components(
+  data,
+  max_labels: int = -1,
+  connectivity: int = 26,
+  return_N: bool = False,
+  delta: Union[int, float] = 0,
+  out_dtype: Optional[Any] = None,
+  out_file=None,
+  periodic_boundary: bool = False,
+  binary_image: bool = False,
+):
+  """Connected components applied to 1D, 2D or 3D images on the GPU.
+
+  Same arguments and results as cc3d.connected_components
+  (cc3d/fastcc3d.pyx:244-626): labels numbered 1..N in order of first
+  appearance in memory (x fastest), output dtype uint16/uint32/uint64 chosen
+  from the provisional-label estimate unless given.  `max_labels` is accepted
+  and ignored, as in the reference.
+  """
+  kind, pl = _prepare(data, max_labels, connectivity, delta, out_dtype, out_file, periodic_boundary,
+                      binary_image, return_N)
+  if kind == "done":
+    return pl
+  return _label_one(pl)
+
+
+def _label_one(pl):
   _require_gpu()
   torch = _torch()
-  buf = vol.device_buffer(order)
-  ws_bytes = int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, int(binary_image), flags))
+  buf = pl.vol.device_buffer(pl.order)
+  sx, sy, sz, connectivity = pl.sx, pl.sy, pl.sz, pl.connectivity
+  ws_bytes = int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, int(pl.binary_image), pl.flags))
   if ws_bytes == 0:
     L.check(L.ERR_RUNTIME if connectivity in (4, 8) and sz != 1 else L.ERR_INVALID)
   ws = torch.empty(ws_bytes, dtype=torch.uint8, device=buf.device)
   stream = _stream()
   job = ctypes.c_void_p(0)
-  dbuf = ctypes.create_string_buffer(delta_b, max(len(delta_b), 8))
+  dbuf = ctypes.create_string_buffer(pl.delta_b, max(len(pl.delta_b), 8))
   L.check(L.lib().cc3d_amd_label_begin(
-    ctypes.byref(job), ctypes.c_void_p(buf.data_ptr()), code, sx, sy, sz, connectivity,
-    ctypes.cast(dbuf, ctypes.c_void_p), flags, int(bool(periodic_boundary)), int(binary_image),
+    ctypes.byref(job), ctypes.c_void_p(buf.data_ptr()), pl.code, sx, sy, sz, connectivity,
+    ctypes.cast(dbuf, ctypes.c_void_p), pl.flags, int(pl.periodic_boundary), int(pl.binary_image),
     ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream))
   try:
     st = L.Stats()
     L.check(L.lib().cc3d_amd_job_stats(job, ctypes.byref(st)))
-    if binary_flag_given:
-      epl, ffr, lfr = voxels, 0, sy
-    else:
-      epl, ffr, lfr = int(st.epl), int(st.first_foreground_row), int(st.last_foreground_row)
-    max_labels = min(epl, voxels)
-    if binary_image:
-      max_labels = binary_bound(max_labels)
-    odt = choose_out_dtype(max_labels)
-
-    special_row = (delta == 0 and ffr == lfr and ffr >= 0)
-    N = int(st.N)
-    if not special_row:
-      uf_len = int(L.lib().cc3d_amd_union_find_length(
-        max_labels, voxels, connectivity, int(binary_image), _OUT_CODE[odt]))
-      if uf_len == 0:
-        N = 0  # the reference returns before labelling (max_labels == 0)
-      elif (not (binary_image and connectivity == 8) or q1_count) and int(st.provisional) >= uf_len:
-        raise RuntimeError("maximum length exception")
-
-    out_dev = torch.empty(voxels, dtype=_torch_out_dtype(odt), device=buf.device)
+    odt, N = _decide(pl, st)
+    out_dev = torch.empty(pl.voxels, dtype=_torch_out_dtype(odt), device=buf.device)
     L.check(L.lib().cc3d_amd_label_finish(job, ctypes.c_void_p(out_dev.data_ptr()), _OUT_CODE[odt]))
   finally:
     L.lib().cc3d_amd_job_free(job)
 
-  return _deliver(out_dev, vol, sx, sy, sz, dims, order, odt, out_file, N, return_N)
+  return _deliver(out_dev, pl.vol, sx, sy, sz, pl.dims, pl.order, odt, pl.out_file, N, pl.return_N)
 
 
 def _deliver(out_dev, vol, sx, sy, sz, dims, order, out_dtype, out_file, N, return_N):

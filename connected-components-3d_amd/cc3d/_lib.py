@@ -28,6 +28,16 @@ class Stats(ctypes.Structure):
   ]
 
 
+class Volume(ctypes.Structure):
+  _fields_ = [
+    ("inp", ctypes.c_void_p),
+    ("sx", ctypes.c_int64),
+    ("sy", ctypes.c_int64),
+    ("sz", ctypes.c_int64),
+    ("connectivity", ctypes.c_int64),
+  ]
+
+
 _lib = None
 
 
@@ -66,6 +76,20 @@ def lib():
   L.cc3d_amd_job_free.argtypes = [vp]
   L.cc3d_amd_union_find_length.restype = u64
   L.cc3d_amd_union_find_length.argtypes = [i64, i64, i64, ci, ci]
+  L.cc3d_amd_batch_workspace_bytes.restype = ctypes.c_size_t
+  L.cc3d_amd_batch_workspace_bytes.argtypes = [ctypes.POINTER(Volume), ci, ci]
+  L.cc3d_amd_batch_create.restype = ci
+  L.cc3d_amd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Volume), ci, ci, ci, vp, ctypes.c_size_t, vp]
+  L.cc3d_amd_batch_set_outputs.restype = ci
+  L.cc3d_amd_batch_set_outputs.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ci)]
+  for name in ("cc3d_amd_batch_begin", "cc3d_amd_batch_finish", "cc3d_amd_batch_graph_capture",
+               "cc3d_amd_batch_graph_launch"):
+    getattr(L, name).restype = ci
+    getattr(L, name).argtypes = [vp]
+  L.cc3d_amd_batch_stats.restype = ci
+  L.cc3d_amd_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+  L.cc3d_amd_batch_free.restype = None
+  L.cc3d_amd_batch_free.argtypes = [vp]
   L.cc3d_amd_last_error.restype = ctypes.c_char_p
   L.cc3d_amd_last_error.argtypes = []
   L.cc3d_amd_version.restype = ctypes.c_char_p

@@ -50,6 +50,22 @@ __global__ void k_init(Counters* c, uint32_t* bits, int64_t nwords) {
   for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
 }
 
+// Batch: reset nc counters and clear the root bitmask (a fill kernel, so the
+// sequence can be captured in a graph).
+__global__ void k_batch_init(Counters* c, int nc, uint32_t* bits, int64_t nwords) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) {
+    c[i].epl = 0;
+    c[i].prov = 0;
+    c[i].first_row = 0x7FFFFFFFFFFFFFFFll;
+    c[i].last_row = -1;
+    c[i].req_count = 0;
+    c[i].req_overflow = 0;
+    c[i].N = 0;
+  }
+  for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
+}
+
 // Folds the local kernel's per-tile counters into Counters.
 __global__ __launch_bounds__(WG) void k_tile_counters(const TileCnt* tc, int64_t ntiles, Counters* c) {
   __shared__ unsigned long long se[WG / 64], sp[WG / 64];
@@ -767,6 +783,7 @@ struct ProfRec {
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<hipEvent_t> g_prof_pool;
 static bool g_prof_on = false;
+static thread_local bool g_prof_suspended = false;  // inside a graph capture
 static std::mutex g_prof_mu;
 
 static hipEvent_t prof_event() {
@@ -785,7 +802,7 @@ struct ProfScope {
   hipStream_t s;
   const char* name;
   ProfScope(const char* n, hipStream_t st) : s(st), name(n) {
-    if (!g_prof_on) return;
+    if (!g_prof_on || g_prof_suspended) return;
     std::lock_guard<std::mutex> g(g_prof_mu);
     a = prof_event();
     (void)hipEventRecord(a, s);
@@ -908,6 +925,25 @@ struct cc3d_amd_job {
   Counters* h_ctr = nullptr;
   bool have_stats = false;
   GlobalParams gp{};
+};
+
+struct cc3d_amd_batch {
+  int n = 0, dt = 0, mode = M_MULTI;
+  hipStream_t stream = nullptr;
+  char* ws = nullptr;
+  bool own_ws = false;
+  size_t total = 0;
+  int64_t ntiles = 0, nkeys = 0, nwords = 0, nblk = 0;
+  size_t oVOL = 0, oT0 = 0, oGP = 0, oRL = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0, oBITS = 0,
+         oPRE = 0, oBSUM = 0, oCTR = 0;
+  BatchVol* h_tab = nullptr;  // pinned host copy of the volume table
+  int64_t* h_t0 = nullptr;    // pinned: first tile of each volume (n + 1)
+  Counters* h_ctr = nullptr;  // pinned: entry 0 whole batch, 1 + v volume v
+  bool outputs_set = false, capturing = false;
+  GlobalParams gp{};
+  LocalParams lp{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -1381,6 +1417,310 @@ int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* c
     n++;
   }
   return n;
+}
+
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vector<BatchVol>& tab,
+                        std::vector<int64_t>& t0, cc3d_amd_batch& b) {
+  if (!vols || n <= 0) return fail(CC3D_AMD_ERR_INVALID, "a batch needs at least one volume");
+  tab.resize(n);
+  t0.resize(n + 1);
+  uint64_t key = 0;
+  int64_t tiles = 0;
+  for (int i = 0; i < n; i++) {
+    const cc3d_amd_volume& v = vols[i];
+    if (!v.in) return fail(CC3D_AMD_ERR_INVALID, "NULL input in batch");
+    if (v.sx <= 0 || v.sy <= 0 || v.sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
+    int nb = (int)v.connectivity;
+    if (nb == 8 || nb == 4) {
+      if (v.sz != 1) return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
+      if (binary) return fail(CC3D_AMD_ERR_INVALID, "binary 2D volumes are not supported in a batch");
+      nb = nb == 8 ? 26 : 6;  // a single plane: the same links
+    } else if (nb == 26 || nb == 18 || nb == 6) {
+      if (binary && v.sz == 1) return fail(CC3D_AMD_ERR_INVALID, "binary 2D volumes are not supported in a batch");
+    } else {
+      return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
+    }
+    BatchVol& t = tab[i];
+    t.in = v.in;
+    t.out = nullptr;
+    t.sx = v.sx;
+    t.sy = v.sy;
+    t.sz = v.sz;
+    t.osx = (v.sx + 1) >> 1;
+    t.ntx = (v.sx + Geo3::TX - 1) / Geo3::TX;
+    t.nty = (v.sy + Geo3::TY - 1) / Geo3::TY;
+    t.ntz = (v.sz + Geo3::TZ - 1) / Geo3::TZ;
+    t.ntiles = t.ntx * t.nty * t.ntz;
+    t.tile0 = tiles;
+    const uint64_t V = (uint64_t)v.sx * (uint64_t)v.sy * (uint64_t)v.sz;
+    t.key0 = (uint32_t)key;
+    t.nkeys = (uint32_t)std::min<uint64_t>(V, 0xFFFFFFFFull);
+    t.nb = nb;
+    t.out_dtype = DT_U32;
+    t0[i] = tiles;
+    tiles += t.ntiles;
+    key += (V + 31) & ~(uint64_t)31;
+    if (key >= 0xFFFFFFFFull) return fail(CC3D_AMD_ERR_INVALID, "a batch must hold fewer than 2^32 - 1 voxels");
+  }
+  t0[n] = tiles;
+  b.n = n;
+  b.ntiles = tiles;
+  b.nkeys = (int64_t)key;
+  b.nwords = (b.nkeys + 31) / 32;
+  b.nblk = (b.nwords + SCAN_WPB - 1) / SCAN_WPB;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += align256(bytes);
+    return r;
+  };
+  const size_t slots = (size_t)tiles * Geo3::TV;
+  b.oVOL = take((size_t)n * sizeof(BatchVol));
+  b.oT0 = take((size_t)(n + 1) * sizeof(int64_t));
+  b.oGP = take((size_t)b.nkeys * 4);
+  b.oRL = take(slots * 4);
+  b.oTR = take((size_t)tiles * 4);
+  b.oTQ = take((size_t)tiles * 4);
+  b.oTC = take((size_t)tiles * sizeof(TileCnt));
+  b.oJ8 = take(slots);
+  b.oJ16 = take(slots * 2);
+  b.oREQ = take((size_t)tiles * Geo3::HALO * 4);
+  b.oBITS = take((size_t)b.nwords * 4);
+  b.oPRE = take((size_t)b.nwords * 4);
+  b.oBSUM = take((size_t)(b.nblk + 1) * 4);
+  b.oCTR = take((size_t)(n + 1) * sizeof(Counters));
+  b.total = o;
+  return CC3D_AMD_OK;
+}
+
+size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int binary_image) {
+  std::vector<BatchVol> tab;
+  std::vector<int64_t> t0;
+  cc3d_amd_batch b;
+  if (batch_layout(vols, n, binary_image, tab, t0, b) != CC3D_AMD_OK) return 0;
+  return b.total;
+}
+
+int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vols, int n, int in_dtype,
+                          int binary_image, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!out_batch) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
+  *out_batch = nullptr;
+  if (in_dtype < DT_U8 || in_dtype > DT_F64) return fail(CC3D_AMD_ERR_INVALID, "unsupported input dtype");
+  std::vector<BatchVol> tab;
+  std::vector<int64_t> t0;
+  cc3d_amd_batch* b = new cc3d_amd_batch();
+  int rc = batch_layout(vols, n, binary_image, tab, t0, *b);
+  if (rc != CC3D_AMD_OK) {
+    delete b;
+    return rc;
+  }
+  b->dt = in_dtype;
+  b->mode = binary_image ? M_BINARY : M_MULTI;
+  b->stream = static_cast<hipStream_t>(stream);
+  if (workspace) {
+    if (workspace_bytes < b->total) {
+      delete b;
+      return fail(CC3D_AMD_ERR_INVALID, "workspace too small");
+    }
+    b->ws = static_cast<char*>(workspace);
+  } else {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, b->total, b->stream) != hipSuccess || !p) {
+      delete b;
+      return fail(CC3D_AMD_ERR_NOMEM, "could not allocate batch workspace");
+    }
+    b->ws = static_cast<char*>(p);
+    b->own_ws = true;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->h_tab), n * sizeof(BatchVol), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_t0), (n + 1) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_ctr), (n + 1) * sizeof(Counters), hipHostMallocDefault) !=
+        hipSuccess) {
+    cc3d_amd_batch_free(b);
+    return fail(CC3D_AMD_ERR_NOMEM, "could not allocate pinned batch tables");
+  }
+  std::memcpy(b->h_tab, tab.data(), n * sizeof(BatchVol));
+  std::memcpy(b->h_t0, t0.data(), (n + 1) * sizeof(int64_t));
+  char* ws = b->ws;
+  if (hipMemcpyAsync(ws + b->oVOL, b->h_tab, n * sizeof(BatchVol), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(ws + b->oT0, b->h_t0, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, b->stream) !=
+        hipSuccess) {
+    cc3d_amd_batch_free(b);
+    return fail(CC3D_AMD_ERR_HIP, "could not upload the batch table");
+  }
+  GlobalParams& gp = b->gp;
+  gp.ntiles = b->ntiles;
+  gp.GP = reinterpret_cast<uint32_t*>(ws + b->oGP);
+  gp.RL = reinterpret_cast<uint32_t*>(ws + b->oRL);
+  gp.TR = reinterpret_cast<uint32_t*>(ws + b->oTR);
+  gp.J8 = reinterpret_cast<uint8_t*>(ws + b->oJ8);
+  gp.J16 = reinterpret_cast<uint16_t*>(ws + b->oJ16);
+  gp.REQ = reinterpret_cast<uint32_t*>(ws + b->oREQ);
+  gp.TQ = reinterpret_cast<uint32_t*>(ws + b->oTQ);
+  gp.bits = reinterpret_cast<uint32_t*>(ws + b->oBITS);
+  gp.prefix = reinterpret_cast<uint32_t*>(ws + b->oPRE);
+  gp.bsum = reinterpret_cast<uint32_t*>(ws + b->oBSUM);
+  gp.nwords = b->nwords;
+  gp.ctr = reinterpret_cast<Counters*>(ws + b->oCTR);
+  gp.vols = reinterpret_cast<const BatchVol*>(ws + b->oVOL);
+  gp.vtile0 = reinterpret_cast<const int64_t*>(ws + b->oT0);
+  gp.nvol = n;
+  LocalParams& lp = b->lp;
+  lp.ntiles = b->ntiles;
+  lp.delta_bits = 0;
+  lp.minmax = nullptr;
+  lp.GP = gp.GP;
+  lp.RL = gp.RL;
+  lp.TR = reinterpret_cast<uint32_t*>(ws + b->oTR);
+  lp.J8 = reinterpret_cast<uint8_t*>(ws + b->oJ8);
+  lp.J16 = reinterpret_cast<uint16_t*>(ws + b->oJ16);
+  lp.REQ = reinterpret_cast<uint32_t*>(ws + b->oREQ);
+  lp.TQ = reinterpret_cast<uint32_t*>(ws + b->oTQ);
+  lp.tcnt = reinterpret_cast<TileCnt*>(ws + b->oTC);
+  *out_batch = b;
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_set_outputs(cc3d_amd_batch* b, void* const* outs, const int* out_dtypes) {
+  if (!b || !outs || !out_dtypes) return fail(CC3D_AMD_ERR_INVALID, "NULL argument");
+  for (int i = 0; i < b->n; i++) {
+    if (!outs[i]) return fail(CC3D_AMD_ERR_INVALID, "NULL output in batch");
+    if (out_dtypes[i] != DT_U16 && out_dtypes[i] != DT_U32 && out_dtypes[i] != DT_U64)
+      return fail(CC3D_AMD_ERR_INVALID, "output dtype must be uint16, uint32 or uint64");
+  }
+  // the pinned table may still feed an earlier upload
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->n; i++) {
+    b->h_tab[i].out = outs[i];
+    b->h_tab[i].out_dtype = out_dtypes[i];
+  }
+  HIPCHECK(hipMemcpyAsync(b->ws + b->oVOL, b->h_tab, b->n * sizeof(BatchVol), hipMemcpyHostToDevice, b->stream));
+  b->outputs_set = true;
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
+  if (!b) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
+  hipStream_t s = b->stream;
+  const GlobalParams& p = b->gp;
+  {
+    ProfScope ps_("k_init", s);
+    hipLaunchKernelGGL(k_batch_init, dim3(grid_for(std::max<int64_t>(b->nwords, b->n + 1), WG, 8192)), dim3(WG), 0, s,
+                       p.ctr, b->n + 1, p.bits, b->nwords);
+  }
+  HIPCHECK(hipGetLastError());
+  hipError_t e;
+  {
+    ProfScope ps_("k_local", s);
+    if (b->mode == M_BINARY) e = launch_local_batch_binary(b->dt, b->lp, p.vols, p.vtile0, b->n, s);
+    else e = launch_local_batch_multi(b->dt, b->lp, p.vols, p.vtile0, b->n, s);
+  }
+  if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_local_batch: ") + hipGetErrorString(e));
+  {
+    ProfScope ps_("k_tile_counters", s);
+    hipLaunchKernelGGL(k_tile_counters_batch, dim3(b->n), dim3(WG), 0, s, b->lp.tcnt, p.vtile0, p.ctr + 1);
+  }
+  HIPCHECK(hipGetLastError());
+  const unsigned wave_blocks = grid_for(b->ntiles, 4, 65536);
+  {
+    ProfScope ps_("k_requests", s);
+    hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  }
+  {
+    ProfScope ps_("k_roots", s);
+    hipLaunchKernelGGL((k_roots<Geo3>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  }
+  {
+    ProfScope ps_("k_scan", s);
+    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)b->nblk, p.ctr);
+    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
+  }
+  {
+    ProfScope ps_("k_labels", s);
+    hipLaunchKernelGGL((k_labels<Geo3>), dim3(wave_blocks), dim3(WG), 0, s, p);
+    hipLaunchKernelGGL(k_batch_N, dim3((b->n + 63) / 64), dim3(64), 0, s, p.vols, b->n, p.prefix, p.nwords, p.ctr,
+                       p.ctr + 1);
+  }
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_stats(cc3d_amd_batch* b, cc3d_amd_stats* st) {
+  if (!b || !st) return fail(CC3D_AMD_ERR_INVALID, "NULL argument");
+  HIPCHECK(hipMemcpyAsync(b->h_ctr, b->ws + b->oCTR, (b->n + 1) * sizeof(Counters), hipMemcpyDeviceToHost, b->stream));
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->n; i++) {
+    const Counters& c = b->h_ctr[i + 1];
+    st[i].N = c.N;
+    st[i].provisional = c.prov;
+    st[i].epl = c.epl;
+    st[i].first_foreground_row = c.last_row < 0 ? -1 : c.first_row;
+    st[i].last_foreground_row = c.last_row;
+  }
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_finish(cc3d_amd_batch* b) {
+  if (!b) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
+  if (!b->outputs_set) return fail(CC3D_AMD_ERR_INVALID, "batch outputs are not set");
+  ProfScope ps_("k_finalize", b->stream);
+  hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)b->ntiles), dim3(WG), 0, b->stream, b->gp);
+  HIPCHECK(hipGetLastError());
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_graph_capture(cc3d_amd_batch* b) {
+  if (!b) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
+  if (b->own_ws) return fail(CC3D_AMD_ERR_INVALID, "graph capture needs a caller-provided workspace");
+  if (!b->outputs_set) return fail(CC3D_AMD_ERR_INVALID, "batch outputs are not set");
+  if (b->stream == nullptr) return fail(CC3D_AMD_ERR_INVALID, "graph capture needs a non-default stream");
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  g_prof_suspended = true;
+  int rc = cc3d_amd_batch_begin(b);
+  if (rc == CC3D_AMD_OK) rc = cc3d_amd_batch_finish(b);
+  g_prof_suspended = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (rc != CC3D_AMD_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = graph;
+  HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  return CC3D_AMD_OK;
+}
+
+int cc3d_amd_batch_graph_launch(cc3d_amd_batch* b) {
+  if (!b || !b->gexec) return fail(CC3D_AMD_ERR_INVALID, "no captured graph");
+  HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+  return CC3D_AMD_OK;
+}
+
+void cc3d_amd_batch_free(cc3d_amd_batch* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->h_tab || b->h_t0 || b->h_ctr) (void)hipStreamSynchronize(b->stream);
+  if (b->own_ws && b->ws) (void)hipFreeAsync(b->ws, b->stream);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  if (b->h_t0) (void)hipHostFree(b->h_t0);
+  if (b->h_ctr) (void)hipHostFree(b->h_ctr);
+  delete b;
 }
 
 }  // extern "C"

@@ -102,6 +102,44 @@ void cc3d_amd_job_free(cc3d_amd_job* job);
 uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t connectivity,
                                     int binary_image, int out_dtype);
 
+/* ---- batches: many volumes labelled by one set of launches ---------------
+ * No reference counterpart: the reference labels one volume per call.  Each
+ * volume's result equals cc3d_amd_connected_components3d on that volume
+ * alone.  Every launch (local labelling, cross-tile unions, root scan,
+ * finalize) covers all tiles of all volumes, so small volumes fill the GPU and
+ * a batch costs one host synchronisation.  Supported per volume: multilabel or
+ * binary (per batch), connectivity 26, 18 or 6 (8 and 4 on a single plane,
+ * multilabel only), no periodic boundary, no delta.  The sum of the volumes'
+ * voxels must stay below 2^32 - 32 * n. */
+typedef struct {
+  const void* in;          /* device pointer, dense, x fastest */
+  int64_t sx, sy, sz;
+  int64_t connectivity;
+} cc3d_amd_volume;
+
+typedef struct cc3d_amd_batch cc3d_amd_batch;
+
+/* Bytes of scratch a batch needs (0 if a volume is unsupported). */
+size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int binary_image);
+/* Plans a batch (no launches).  workspace may be NULL (allocated here and
+ * freed by cc3d_amd_batch_free).  All later launches go to `stream`. */
+int cc3d_amd_batch_create(cc3d_amd_batch** batch, const cc3d_amd_volume* vols, int n, int in_dtype,
+                          int binary_image, void* workspace, size_t workspace_bytes, void* stream);
+/* Output buffer and dtype (1/2/3) of every volume; uploaded stream-ordered. */
+int cc3d_amd_batch_set_outputs(cc3d_amd_batch* batch, void* const* outs, const int* out_dtypes);
+/* Stage 1 for all volumes (labels, counters); no host synchronisation. */
+int cc3d_amd_batch_begin(cc3d_amd_batch* batch);
+/* Waits for the stream and returns n stats records. */
+int cc3d_amd_batch_stats(cc3d_amd_batch* batch, cc3d_amd_stats* stats);
+/* Stage 2 for all volumes into the buffers set by cc3d_amd_batch_set_outputs. */
+int cc3d_amd_batch_finish(cc3d_amd_batch* batch);
+/* Captures begin + finish into a HIP graph (needs a caller workspace and
+ * outputs already set); cc3d_amd_batch_graph_launch replays it on the stream.
+ * Replays read the current contents of the inputs and the output table. */
+int cc3d_amd_batch_graph_capture(cc3d_amd_batch* batch);
+int cc3d_amd_batch_graph_launch(cc3d_amd_batch* batch);
+void cc3d_amd_batch_free(cc3d_amd_batch* batch);
+
 /* ---- follow-ons: statistics and dust ------------------------------------ */
 
 /* Max (unsigned view) or signed min/max of a label volume in HBM; used for
