@@ -43,6 +43,10 @@ NSEEDS = 12000
 SEED = 512
 
 
+def _progress(msg: str) -> None:
+  print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def _profile_rows(lib):
   names = ctypes.create_string_buffer(64 * 64)
   ms = (ctypes.c_double * 64)()
@@ -80,6 +84,7 @@ def batched_leg(torch, cc3d, synth, dev, reps=3, cpu=True):
     conns.append(int(rng.choice([6, 26])))
   gen_s = time.time() - t0
   V = int(sum(v.size for v in vols))
+  _progress(f"batched: generated {len(vols)} volumes, {V} voxels in {gen_s:.1f} s")
   dvols = [torch.from_numpy(np.ravel(v, order="F")).to(dev).as_strided(v.shape, (1, v.shape[0], v.shape[0] * v.shape[1]))
            for v in vols]
   torch.cuda.synchronize()
@@ -95,6 +100,7 @@ def batched_leg(torch, cc3d, synth, dev, reps=3, cpu=True):
     plain()
   torch.cuda.synchronize()
   plain_s = (time.perf_counter() - t0) / reps
+  _progress(f"batched: plain per-volume {plain_s * 1e3:.1f} ms")
 
   lb = cc3d.LabelBatch(dvols, connectivity=conns, out_dtype=np.uint32)
   lb.run(deliver=False)
@@ -113,6 +119,7 @@ def batched_leg(torch, cc3d, synth, dev, reps=3, cpu=True):
     Ns = lb.replay(deliver=False)
   torch.cuda.synchronize()
   graph_s = (time.perf_counter() - t0) / reps
+  _progress(f"batched: batch {batch_s * 1e3:.1f} ms, graph {graph_s * 1e3:.1f} ms")
   # bit-exact check of the batch outputs against the plain path (both in memory order)
   exact = all(int(N) == int(r[1]) for N, r in zip(Ns, ref))
   exact = exact and all(torch.equal(o, r[0].as_strided((r[0].numel(),), (1,))) for o, r in zip(lb.outs, ref))
@@ -137,6 +144,7 @@ def batched_leg(torch, cc3d, synth, dev, reps=3, cpu=True):
       with ThreadPoolExecutor(max_workers=threads) as ex:
         cpuN = list(ex.map(one, range(len(vols))))
       cpu_s = time.perf_counter() - t0
+      _progress(f"batched: reference on {threads} host threads {cpu_s:.2f} s")
       cpu_res = {"value": V / cpu_s / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "reference",
                  "sample": f"the whole batch, one volume per host thread ({cpu_s:.2f} s; the reference releases "
                            "the GIL, so threads run like one process per core)",
@@ -155,6 +163,92 @@ def batched_leg(torch, cc3d, synth, dev, reps=3, cpu=True):
   }
 
 
+def _dev_volume(torch, v, dev):
+  sx, sy, sz = v.shape
+  return torch.from_numpy(np.ravel(v, order="F")).to(dev).as_strided(v.shape, (1, sx, sx * sy))
+
+
+def served_leg(torch, cc3d, synth, dev, nreq=160, loads=(0.5, 0.8, 0.95)):
+  """Open-loop Poisson arrivals on the labelling service: 90 % small volumes
+  (sides U[128, 256], 26-connected), 10 % 1024x1024x512 (200,000 seeds,
+  6-connected); each request also gets device-side voxel counts, bounding boxes
+  and dust below 100 voxels.  Capacity is measured closed-loop first; the
+  plain form serves the same requests one at a time without batching."""
+  rng = np.random.default_rng(4)
+  t0 = time.time()
+  small = []
+  for i in range(24):
+    shape = tuple(int(s) for s in rng.integers(128, 257, size=3))
+    small.append(_dev_volume(torch, synth.voronoi(shape, max(1, int(np.prod(shape)) // 8000), seed=40000 + i), dev))
+  _progress("served: small volumes generated")
+  large = _dev_volume(torch, synth.voronoi((1024, 1024, 512), 200000, seed=41000), dev)
+  gen_s = time.time() - t0
+  _progress(f"served: volumes generated in {gen_s:.1f} s")
+  torch.cuda.synchronize()
+
+  def draw(n, seed):
+    r = np.random.default_rng(seed)
+    return [(small[int(r.integers(len(small)))], 26) if r.random() < 0.9 else (large, 6) for _ in range(n)]
+
+  def burst(srv, reqs):
+    t0 = time.perf_counter()
+    rs = [srv.submit(v, c, t_arrive=t0) for v, c in reqs]
+    for r in rs:
+      if not r.done.wait(120):
+        raise TimeoutError("a served request did not finish within 120 s")
+      if r.error is not None:
+        raise r.error
+    return rs, time.perf_counter() - t0
+
+  vox = lambda reqs: sum(int(v.numel()) for v, _ in reqs)
+  srv = cc3d.LabelServer(workers=4, max_batch=16)
+  burst(srv, draw(12, 1))  # warm-up
+  _progress("served: warm-up done")
+  reqs = draw(nreq, 2)
+  rs, cap_s = burst(srv, reqs)
+  cap = nreq / cap_s
+  _progress(f"served: capacity {cap:.1f} requests/s")
+  plain = cc3d.LabelServer(workers=1, max_batch=1)
+  burst(plain, draw(6, 1))
+  _, plain_s = burst(plain, reqs)
+  plain.close()
+  _progress(f"served: plain {nreq / plain_s:.1f} requests/s")
+  levels = []
+  for f in loads:
+    lam = f * cap
+    r = np.random.default_rng(int(f * 1000))
+    arrivals = np.cumsum(r.exponential(1.0 / lam, size=nreq))
+    reqs = draw(nreq, 3 + int(f * 100))
+    base = time.perf_counter() + 0.01
+    rs = []
+    for a, (v, c) in zip(arrivals, reqs):
+      now = time.perf_counter()
+      if base + a > now:
+        time.sleep(base + a - now)
+      rs.append(srv.submit(v, c, t_arrive=base + a))
+    for q in rs:
+      if not q.done.wait(120):
+        raise TimeoutError("a served request did not finish within 120 s")
+      if q.error is not None:
+        raise q.error
+    end = max(q.t_done for q in rs)
+    _progress(f"served: load {f} done")
+    lat = np.array([q.latency for q in rs]) * 1e3
+    levels.append({"load": f, "offered_rps": lam, "achieved_rps": nreq / (end - base),
+                   "gvox_s": vox(reqs) / (end - base) / 1e9,
+                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "mean_batch": float(np.mean([q.batched_with for q in rs]))})
+  srv.close()
+  return {
+    "workload": "Poisson arrivals; 90% uint32 Voronoi sides U[128,256] 26-conn, 10% 1024x1024x512 200k seeds "
+                "6-conn; each + device counts, bboxes, dust<100; inputs resident in HBM",
+    "requests_per_level": nreq,
+    "capacity_rps": cap, "capacity_gvox_s": vox(draw(nreq, 2)) / cap_s / 1e9,
+    "plain_rps": nreq / plain_s, "served_vs_plain": plain_s / cap_s,
+    "levels": levels, "generate_s": gen_s,
+  }
+
+
 def main() -> int:
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +256,7 @@ def main() -> int:
   ap.add_argument("--warmup", type=int, default=3)
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
   ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+  ap.add_argument("--no-served", action="store_true", help="skip the served leg")
   ap.add_argument("--calib", action="store_true",
                   help="also run one 256 MiB device copy (counter calibration for profiles)")
   args = ap.parse_args()
@@ -181,6 +276,7 @@ def main() -> int:
   vol = synth.voronoi(SHAPE, NSEEDS, seed=SEED)
   gen_s = time.time() - t0
   V = vol.size
+  _progress(f"headline: volume generated in {gen_s:.1f} s")
   flat = torch.from_numpy(np.ravel(vol, order="F")).to(dev)
   sx, sy, sz = SHAPE
   dvol = flat.as_strided(SHAPE, (1, sx, sx * sy))
@@ -215,6 +311,7 @@ def main() -> int:
   elapsed = time.perf_counter() - t0
   lib.cc3d_amd_profile_enable(0)
   rows = _profile_rows(lib)
+  _progress(f"headline: {args.steps} steps in {elapsed * 1e3:.1f} ms")
 
   ms_per_step = elapsed * 1e3 / args.steps
   gvox = V * args.steps / elapsed / 1e9
@@ -256,6 +353,7 @@ def main() -> int:
       ref_out, refN = oracle.connected_components(vol, connectivity=26, out_dtype=np.uint32, return_N=True,
                                                   backend="ref")
       cpu_s = time.perf_counter() - t0
+      _progress(f"headline: reference on one host core {cpu_s:.2f} s")
       got = out.cpu().numpy()
       exact = bool(refN == N and np.array_equal(got, ref_out))
       cpu = {"value": V / cpu_s / 1e9, "unit": "Gvoxels/s", "cores": 1, "kind": "reference",
@@ -265,6 +363,11 @@ def main() -> int:
   batched = None
   if not args.no_batched:
     batched = batched_leg(torch, cc3d, synth, dev, cpu=not args.no_cpu)
+
+  served = None
+  if not args.no_served:
+    torch.cuda.empty_cache()
+    served = served_leg(torch, cc3d, synth, dev)
 
   line = {
     "metric": METRIC,
@@ -288,6 +391,7 @@ def main() -> int:
     "cpu_baseline": cpu,
     "bit_exact_vs_reference": exact,
     "batched": batched,
+    "served": served,
     "generate_s": gen_s,
   }
   print(json.dumps(line))

@@ -8,6 +8,7 @@ signatures and results, computed by hand-written HIP kernels for gfx950
 from ._batch import LabelBatch, connected_components_batch
 from ._core import DimensionError, connected_components, estimate_provisional_labels
 from ._follow import dust, statistics
+from ._serve import LabelServer, Request
 
 __version__ = "4.0.0+mi355x.0.1"
 
@@ -19,4 +20,5 @@ __all__ = [
   "DimensionError",
   "connected_components_batch",
   "LabelBatch",
+  "LabelServer",
 ]

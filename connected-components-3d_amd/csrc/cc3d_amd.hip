@@ -413,10 +413,12 @@ __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
 
 // Stage 2: every voxel of the tile gets its local root's label.  Thread t
 // owns the column the local kernel gave it: its 8 root indices are one 8- or
-// 16-byte load.
+// 16-byte load, issued together with a coalesced copy of the tile's label
+// table into LDS, so the per-voxel lookups hit LDS instead of scattered
+// global reads.
 template <class OUT, class G>
 __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_t tile, const VolGeo& v,
-                                     OUT* __restrict__ out) {
+                                     OUT* __restrict__ out, uint32_t* lds_labels) {
   static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
   const int t = threadIdx.x;
   const int64_t tz = tile / (v.ntx * v.nty);
@@ -424,8 +426,9 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
   const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
   const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
   const uint32_t* __restrict__ LT = p.RL + slot * G::TV;
+  const uint32_t R = p.TR[slot];
   uint32_t j[8];
-  if (p.TR[slot] <= 255) {
+  if (R <= 255) {
     const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + slot * G::TV)[t];
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
@@ -435,9 +438,9 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
-  uint32_t lab[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) lab[k] = j[k] ? LT[j[k] - 1] : 0u;
+  lds_labels[0] = 0u;  // index 0: background
+  for (uint32_t i = t; i < R; i += WG) lds_labels[i + 1] = LT[i];
+  __syncthreads();
   int lx, ly, lz;
   G::voxel(t, 0, lx, ly, lz);
   const int64_t gx = x0 + lx;
@@ -446,25 +449,30 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
   for (int k = 0; k < 8; k++) {
     G::voxel(t, k, lx, ly, lz);
     const int64_t gy = y0 + ly, gz = z0 + lz;
-    if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[k];
+    if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lds_labels[j[k]];
   }
 }
 
 template <class OUT, class G>
 __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
+  __shared__ uint32_t lab[G::TV + 1];
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
-  finalize_tile<OUT, G>(p, tile, tile, vol_geo(p, 0), out);
+  finalize_tile<OUT, G>(p, tile, tile, vol_geo(p, 0), out, lab);
 }
 
 // Batch: each tile writes its own volume's output in that volume's dtype.
 template <class G>
 __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
+  __shared__ uint32_t lab[G::TV + 1];
   const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
   const BatchVol& bv = p.vols[find_vol(p.vtile0, p.nvol, slot)];
   const VolGeo v = vol_geo(p, slot);
-  if (bv.out_dtype == DT_U16) finalize_tile<uint16_t, G>(p, slot, slot - v.tile0, v, static_cast<uint16_t*>(bv.out));
-  else if (bv.out_dtype == DT_U64) finalize_tile<uint64_t, G>(p, slot, slot - v.tile0, v, static_cast<uint64_t*>(bv.out));
-  else finalize_tile<uint32_t, G>(p, slot, slot - v.tile0, v, static_cast<uint32_t*>(bv.out));
+  if (bv.out_dtype == DT_U16)
+    finalize_tile<uint16_t, G>(p, slot, slot - v.tile0, v, static_cast<uint16_t*>(bv.out), lab);
+  else if (bv.out_dtype == DT_U64)
+    finalize_tile<uint64_t, G>(p, slot, slot - v.tile0, v, static_cast<uint64_t*>(bv.out), lab);
+  else
+    finalize_tile<uint32_t, G>(p, slot, slot - v.tile0, v, static_cast<uint32_t*>(bv.out), lab);
 }
 
 // Batch: per-volume counters from the per-tile records (one block per volume).

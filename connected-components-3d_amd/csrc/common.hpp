@@ -299,12 +299,15 @@ __device__ inline uint32_t lunion(uint32_t* par, uint32_t a, uint32_t b) {
 __device__ inline uint32_t gload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Path halving with plain (agent-scope) stores: a node that is not a root is
+// never touched by a union again, and every halving store writes one of its
+// ancestors, so a concurrent reader sees either value on a valid path.
 __device__ inline uint32_t gfind(uint32_t* GP, uint32_t a) {
   uint32_t p = gload(GP + a);
   while (p != a) {
     const uint32_t gp = gload(GP + p);
     if (gp == p) return p;
-    atomicMin(GP + a, gp);  // path halving, as in lfind_h
+    __hip_atomic_store(GP + a, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     a = gp;
     p = gload(GP + a);
   }
