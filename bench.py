@@ -192,16 +192,24 @@ def served_leg(torch, cc3d, synth, dev, nreq=160, loads=(0.5, 0.8, 0.95)):
 
   def burst(srv, reqs):
     t0 = time.perf_counter()
-    rs = [srv.submit(v, c, t_arrive=t0) for v, c in reqs]
+    rs = [srv.submit(v, c, t_arrive=t0, keep=False) for v, c in reqs]
     for r in rs:
       if not r.done.wait(120):
         raise TimeoutError("a served request did not finish within 120 s")
       if r.error is not None:
         raise r.error
+      r.result = None  # the caller is done with it
     return rs, time.perf_counter() - t0
 
   vox = lambda reqs: sum(int(v.numel()) for v, _ in reqs)
   srv = cc3d.LabelServer(workers=4, max_batch=16)
+  try:
+    return _served_runs(srv, cc3d, draw, burst, vox, nreq, loads, gen_s)
+  finally:
+    srv.close()
+
+
+def _served_runs(srv, cc3d, draw, burst, vox, nreq, loads, gen_s):
   burst(srv, draw(12, 1))  # warm-up
   _progress("served: warm-up done")
   reqs = draw(nreq, 2)
@@ -225,12 +233,13 @@ def served_leg(torch, cc3d, synth, dev, nreq=160, loads=(0.5, 0.8, 0.95)):
       now = time.perf_counter()
       if base + a > now:
         time.sleep(base + a - now)
-      rs.append(srv.submit(v, c, t_arrive=base + a))
+      rs.append(srv.submit(v, c, t_arrive=base + a, keep=False))
     for q in rs:
       if not q.done.wait(120):
         raise TimeoutError("a served request did not finish within 120 s")
       if q.error is not None:
         raise q.error
+      q.result = None
     end = max(q.t_done for q in rs)
     _progress(f"served: load {f} done")
     lat = np.array([q.latency for q in rs]) * 1e3
@@ -238,7 +247,6 @@ def served_leg(torch, cc3d, synth, dev, nreq=160, loads=(0.5, 0.8, 0.95)):
                    "gvox_s": vox(reqs) / (end - base) / 1e9,
                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                    "mean_batch": float(np.mean([q.batched_with for q in rs]))})
-  srv.close()
   return {
     "workload": "Poisson arrivals; 90% uint32 Voronoi sides U[128,256] 26-conn, 10% 1024x1024x512 200k seeds "
                 "6-conn; each + device counts, bboxes, dust<100; inputs resident in HBM",

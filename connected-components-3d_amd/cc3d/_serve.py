@@ -21,7 +21,8 @@ Scheduling:
 
 Footprint of a request (an upper bound, reserved at admission and released at
 completion): labelling workspace + uint32 labels + the dusted copy + the
-statistics arrays for up to one label per voxel.
+statistics arrays for up to one label per voxel.  Results belong to the caller
+after completion; the budget should leave room for the results callers keep.
 """
 from __future__ import annotations
 
@@ -56,6 +57,7 @@ class Request:
     self.footprint = 0
     self.batched_with = 1
     self.ready = None
+    self.keep = True
 
   @property
   def latency(self) -> float:
@@ -104,7 +106,7 @@ class LabelServer:
     torch = _torch()
     if hbm_budget is None:
       free, _ = torch.cuda.mem_get_info()
-      hbm_budget = int(free * 0.6)
+      hbm_budget = int(free * 0.5)
     self.budget = int(hbm_budget)
     self.free = self.budget
     self.max_batch = int(max_batch)
@@ -113,21 +115,22 @@ class LabelServer:
     self.cv = threading.Condition()
     self.waiting = collections.deque()
     self.closing = False
-    self.completed: List[Request] = []
     self.streams = [torch.cuda.Stream() for _ in range(workers)]
     self.threads = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in self.streams]
     for th in self.threads:
       th.start()
 
-  def submit(self, volume, connectivity: int = 26, t_arrive: Optional[float] = None) -> Request:
+  def submit(self, volume, connectivity: int = 26, t_arrive: Optional[float] = None, keep: bool = True) -> Request:
     """volume: an integer device tensor of shape (sx, sy, sz) with x fastest
-    (strides 1, sx, sx*sy).  t_arrive defaults to now."""
+    (strides 1, sx, sx*sy).  t_arrive defaults to now.  keep=False drops the
+    result tensors at completion (only N and dust_N are kept)."""
     sx, sy, sz = _dims(volume)
     if volume.device.type == "cpu" or tuple(volume.stride()) != (1, sx, sx * sy)[:volume.dim()]:
       raise ValueError("requests take device tensors with x fastest (strides 1, sx, sx*sy)")
     r = Request(volume, connectivity, time.perf_counter() if t_arrive is None else t_arrive)
     r.ready = _torch().cuda.Event()
     r.ready.record()  # the volume's producers on the caller's stream come first
+    r.keep = bool(keep)
     r.footprint = request_footprint(volume, connectivity)
     if r.footprint > self.budget:
       r.error = MemoryError(f"request needs {r.footprint} bytes of HBM; the budget is {self.budget}")
@@ -201,7 +204,8 @@ class LabelServer:
         for r, out in zip(group, res):
           nm = int(out.pop("_nmask").item())
           out["dust_N"] = out["N"] - nm
-          r.result = out
+          r.result = out if r.keep else {"N": out["N"], "dust_N": out["dust_N"]}
+        del res, lab
       except BaseException as e:  # reported on the request
         for r in group:
           r.error = e
@@ -210,7 +214,6 @@ class LabelServer:
         for r in group:
           r.t_done = t1
           self.free += r.footprint
-          self.completed.append(r)
         self.cv.notify_all()
       for r in group:
         r.done.set()

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
       const uint32_t g = p.RL[tile * G::TV + j];
       const uint32_t r = gfind(p.GP, g);
       if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
-      else p.GP[g] = r;
+      else atomicMin(p.GP + g, r);  // never above a value another find stored
     }
   }
 }

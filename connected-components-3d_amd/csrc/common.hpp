@@ -299,15 +299,16 @@ __device__ inline uint32_t lunion(uint32_t* par, uint32_t a, uint32_t b) {
 __device__ inline uint32_t gload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Path halving with plain (agent-scope) stores: a node that is not a root is
-// never touched by a union again, and every halving store writes one of its
-// ancestors, so a concurrent reader sees either value on a valid path.
+// Path halving with atomicMin to the grandparent: parents only ever decrease,
+// so a node that k_roots has pointed straight at its root (the minimum of its
+// set) can never be moved back to an intermediate ancestor by a concurrent
+// find on a longer path.
 __device__ inline uint32_t gfind(uint32_t* GP, uint32_t a) {
   uint32_t p = gload(GP + a);
   while (p != a) {
     const uint32_t gp = gload(GP + p);
     if (gp == p) return p;
-    __hip_atomic_store(GP + a, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicMin(GP + a, gp);
     a = gp;
     p = gload(GP + a);
   }

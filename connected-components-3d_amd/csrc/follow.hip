@@ -79,8 +79,6 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, int64_t s0, int64_t 
                                               unsigned long long* sums) {
   const int64_t n = s0 * s1 * s2;
   const int64_t start = ((int64_t)blockIdx.x * WG + threadIdx.x) * STAT_RUN;
-  if (start >= n) return;
-  const int64_t end = start + STAT_RUN < n ? start + STAT_RUN : n;
   auto coords = [&](int64_t i, int64_t& c0, int64_t& c1, int64_t& c2) {
     if (f_order) {
       c0 = i % s0;
@@ -94,11 +92,26 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, int64_t s0, int64_t 
       c0 = r / s1;
     }
   };
-  uint64_t cur = (uint64_t)lab[start];
+  uint64_t cur = start < n ? (uint64_t)lab[start] : 0;
   uint32_t cnt = 0;
   int64_t mn[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx[3] = {-1, -1, -1};
   unsigned long long sm[3] = {0, 0, 0};
+  // Label 0 (background) is in most runs of most volumes: it is accumulated
+  // per thread and reduced per block, so it costs one set of atomics per
+  // block instead of one per run on a single hot address.
+  uint32_t cnt0 = 0;
+  int64_t mn0[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx0[3] = {-1, -1, -1};
+  unsigned long long sm0[3] = {0, 0, 0};
   auto flush = [&]() {
+    if (cur == 0) {
+      cnt0 += cnt;
+      for (int a = 0; a < 3; a++) {
+        mn0[a] = mn[a] < mn0[a] ? mn[a] : mn0[a];
+        mx0[a] = mx[a] > mx0[a] ? mx[a] : mx0[a];
+        sm0[a] += sm[a];
+      }
+      return;
+    }
     atomicAdd(counts + cur, cnt);
     for (int a = 0; a < nd; a++) {
       atomicMin(bbox + 2 * nd * cur + 2 * a, (uint32_t)mn[a]);
@@ -106,28 +119,73 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, int64_t s0, int64_t 
       atomicAdd(sums + nd * cur + a, sm[a]);
     }
   };
-  for (int64_t i = start; i < end; i++) {
-    const uint64_t l = (uint64_t)lab[i];
-    if (l != cur) {
-      flush();
-      cur = l;
-      cnt = 0;
-      for (int a = 0; a < 3; a++) {
-        mn[a] = INT64_MAX;
-        mx[a] = -1;
-        sm[a] = 0;
+  if (start < n) {
+    const int64_t end = start + STAT_RUN < n ? start + STAT_RUN : n;
+    for (int64_t i = start; i < end; i++) {
+      const uint64_t l = (uint64_t)lab[i];
+      if (l != cur) {
+        flush();
+        cur = l;
+        cnt = 0;
+        for (int a = 0; a < 3; a++) {
+          mn[a] = INT64_MAX;
+          mx[a] = -1;
+          sm[a] = 0;
+        }
+      }
+      int64_t c[3];
+      coords(i, c[0], c[1], c[2]);
+      cnt++;
+      for (int a = 0; a < nd; a++) {
+        mn[a] = c[a] < mn[a] ? c[a] : mn[a];
+        mx[a] = c[a] > mx[a] ? c[a] : mx[a];
+        sm[a] += (unsigned long long)c[a];
       }
     }
-    int64_t c[3];
-    coords(i, c[0], c[1], c[2]);
-    cnt++;
-    for (int a = 0; a < nd; a++) {
-      mn[a] = c[a] < mn[a] ? c[a] : mn[a];
-      mx[a] = c[a] > mx[a] ? c[a] : mx[a];
-      sm[a] += (unsigned long long)c[a];
+    flush();
+  }
+  // block reduction of the background accumulators
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt0 += __shfl_xor(cnt0, o);
+    for (int a = 0; a < 3; a++) {
+      const int64_t m1 = __shfl_xor(mn0[a], o), m2 = __shfl_xor(mx0[a], o);
+      mn0[a] = m1 < mn0[a] ? m1 : mn0[a];
+      mx0[a] = m2 > mx0[a] ? m2 : mx0[a];
+      sm0[a] += __shfl_xor(sm0[a], o);
     }
   }
-  flush();
+  __shared__ uint32_t bc[WG / 64];
+  __shared__ long long bmn[WG / 64][3], bmx[WG / 64][3];
+  __shared__ unsigned long long bsm[WG / 64][3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    bc[wv] = cnt0;
+    for (int a = 0; a < 3; a++) {
+      bmn[wv][a] = mn0[a];
+      bmx[wv][a] = mx0[a];
+      bsm[wv][a] = sm0[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < WG / 64; w++) {
+      cnt0 += bc[w];
+      for (int a = 0; a < 3; a++) {
+        mn0[a] = bmn[w][a] < mn0[a] ? bmn[w][a] : mn0[a];
+        mx0[a] = bmx[w][a] > mx0[a] ? bmx[w][a] : mx0[a];
+        sm0[a] += bsm[w][a];
+      }
+    }
+    if (cnt0) {
+      atomicAdd(counts, cnt0);
+      for (int a = 0; a < nd; a++) {
+        atomicMin(bbox + 2 * a, (uint32_t)mn0[a]);
+        atomicMax(bbox + 2 * a + 1, (uint32_t)mx0[a]);
+        atomicAdd(sums + a, sm0[a]);
+      }
+    }
+  }
 }
 
 __global__ void k_centroids(const uint32_t* counts, const unsigned long long* sums, double* cen, int64_t n1,

@@ -213,3 +213,23 @@ def test_512_cube_all_distinct(gpu):
   assert N == 512 ** 3 - 1
   assert out.dtype == np.uint32
   assert np.array_equal(out.ravel(), np.arange(512 ** 3, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("shape,seed", [((128, 256, 384), 30001), ((192, 288, 128), 30030), ((64, 384, 384), 30080)])
+def test_voronoi_6conn_large_repeatable(gpu, shape, seed):
+  """Regression: large 6-connected Voronoi volumes (deep global forests) must
+  match the reference on every one of several runs, as a numpy input and as
+  a device tensor."""
+  import torch
+  import cc3d
+  import synth
+  vol = synth.voronoi(shape, max(1, int(np.prod(shape)) // 8000), seed=seed)
+  backend = "ref" if oracle.have_ref() else "oracle"
+  exp, expN = oracle.connected_components(vol, connectivity=6, out_dtype=np.uint32, return_N=True, backend=backend)
+  dvol = torch.from_numpy(np.ravel(vol, order="F")).cuda().as_strided(shape, (1, shape[0], shape[0] * shape[1]))
+  for run in range(3):
+    print(f"run {run} shape={shape} seed={seed}", flush=True)
+    got, N = cc3d.connected_components(vol, connectivity=6, out_dtype=np.uint32, return_N=True)
+    assert N == expN and np.array_equal(got, exp), f"numpy input, run {run}"
+    got, N = cc3d.connected_components(dvol, connectivity=6, out_dtype=np.uint32, return_N=True)
+    assert N == expN and np.array_equal(got.cpu().numpy(), exp), f"device input, run {run}"
