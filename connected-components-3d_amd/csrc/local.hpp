@@ -87,6 +87,19 @@ __host__ __device__ constexpr int node_off(int i) {
   return o.dx + G::TX * (o.dy + G::TY * o.dz);
 }
 
+// Offsets of the backward positions along one axis, packed as (d + 1) in 2-bit
+// fields, for run-time position indices.
+template <bool IS3D>
+__host__ __device__ constexpr uint32_t packed_offsets(int axis) {
+  uint32_t m = 0;
+  for (int i = 0; i < NbSets<IS3D>::N; i++) {
+    const Off o = NbSets<IS3D>::off(i);
+    const int d = axis == 0 ? o.dx : axis == 1 ? o.dy : o.dz;
+    m |= (uint32_t)(d + 1) << (2 * i);
+  }
+  return m;
+}
+
 // Continuous 2D-8 shortcut (cc3d/cc3d_continuous.hpp:341-349): both values
 // within delta of the global minimum and maximum.
 template <class T>
@@ -132,6 +145,16 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
     return G::template node_box<QK>(bx + o.dx, by + o.dy, bz + o.dz);
   };
   auto cover = [&](int i) -> uint32_t { return HALO ? NH::adj(i) : cover_from<NB>(i, bd); };
+  // node of position i for a run-time i (offsets unpacked from 2-bit fields)
+  auto nd_dyn = [&](int i) -> uint32_t {
+    constexpr uint32_t PX = packed_offsets<IS3D>(0), PY = packed_offsets<IS3D>(1), PZ = packed_offsets<IS3D>(2);
+    const int dx = (int)((PX >> (2 * i)) & 3u) - 1;
+    const int dy = (int)((PY >> (2 * i)) & 3u) - 1;
+    const int dz = (int)((PZ >> (2 * i)) & 3u) - 1;
+    if (HALO) return G::TV + G::halo_index(bx + dx, by + dy, bz + dz);
+    if (QK != Q_BLOCK8 && !bd.H) return pn + dx + G::TX * (dy + G::TY * dz);
+    return G::template node_box<QK>(bx + dx, by + dy, bz + dz);
+  };
   auto ok = [&](int i) -> bool { return !HALO || ((elig >> i) & 1u); };
   uint32_t a = pn;
 
@@ -208,33 +231,41 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
       return (int)lK + (int)lM;
     }
   } else if constexpr (MODE != M_CONT) {
-    // 26 / 18 / 8: transitive links, adjacency covering.  A covered neighbour
-    // that is linked is connected to p as well, so its cover counts too.
-    // Pass 1 visits the neighbours the column link already covers, pass 2 the
-    // rest; only linked neighbours still uncovered when visited are unioned.
+    // 26 / 18 / 8: transitive links.  Read every neighbour into a link mask,
+    // then grow the set of neighbours known to be connected to p: a linked
+    // neighbour adjacent (through a pair unioned in this tile's structure) to
+    // a connected one is connected too.  Only a linked neighbour the growth
+    // cannot reach is unioned, and the growth restarts from it.
     constexpr uint32_t ALLN = (1u << NH::N) - 1u;
     const uint32_t okm = HALO ? (elig & ALLN) : ALLN;
-    uint32_t cov = col ? (cover(0) | 1u) : 0u;
-    uint32_t seen = col ? 1u : 0u;
-    auto lnk = [&](T q) -> bool { return MODE == M_MULTI ? (q == cur) : fg(q); };
-    if (col) {
-#pragma unroll
-      for (int i = 1; i < NH::N; i++) {
-        if (!((okm >> i) & 1u) || !((cov >> i) & 1u)) continue;
-        seen |= 1u << i;
-        if (lnk(val(i))) cov |= cover(i);
-      }
-    }
+    uint32_t lk = 0;
 #pragma unroll
     for (int i = 0; i < NH::N; i++) {
-      if (((cov | seen | ~okm) & ALLN) == ALLN) break;
-      if (!((okm >> i) & 1u) || ((seen >> i) & 1u)) continue;
+      if (col && i == 0) continue;
+      if (!((okm >> i) & 1u)) continue;
       const T q = val(i);
-      if (!lnk(q)) continue;
-      if (!((cov >> i) & 1u)) a = lunion(par, a, nd(i));
-      cov |= cover(i);
+      const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
+      lk |= l ? (1u << i) : 0u;
     }
-    return cov ? 1 : 0;
+    uint32_t conn = col ? 1u : 0u;
+    lk |= conn;
+    uint32_t front = conn;
+    while (true) {
+      while (front) {
+        uint32_t nb = 0;
+#pragma unroll
+        for (int j = 0; j < NH::N; j++) nb |= ((front >> j) & 1u) ? cover(j) : 0u;
+        front = nb & lk & ~conn;
+        conn |= front;
+      }
+      const uint32_t rest = lk & ~conn;
+      if (!rest) break;
+      const int i = __builtin_ctz(rest);
+      a = lunion(par, a, nd_dyn(i));
+      conn |= 1u << i;
+      front = 1u << i;
+    }
+    return lk ? 1 : 0;
   } else {
     // continuous: exact-equality covering plus direct matches with neighbour
     // 0.  Covered neighbours are still read: an exactly equal one extends the
@@ -366,19 +397,76 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   const int b0 = G::box(lx + 1, ly0 + 1, IS3D ? 1 : 0);
 
   // 1. stage the column (registers + LDS) and the halo; voxels outside the
-  //    volume read as background.  Column runs get their parent here.
+  //    volume read as background.  Column runs get their parent here.  All
+  //    global loads (column and halo) are issued before any value is used:
+  //    out-of-volume voxels load a valid address and are replaced by
+  //    background afterwards, so one wait covers every load.
+  constexpr int HLEN = G::TZ > G::BY ? G::TZ : G::BY;
+  constexpr int HV = HRUNS ? HLEN : HPT;
   T col[PT];
+  T hv[HV];
   uint32_t runs = 0;  // bit k: voxel k continues the run of voxel k-1
+  bool inb[PT];
+#pragma unroll
+  for (int k = 0; k < PT; k++) {
+    int vx, vy, vz;
+    G::voxel(t, k, vx, vy, vz);
+    const int64_t gy = y0 + vy, gz = z0 + vz;
+    inb[k] = gx < sx && gy < sy && gz < sz;
+    col[k] = in[inb[k] ? gx + sx * (gy + sy * gz) : 0];
+  }
+  // 3D halo as columns (HRUNS): ring position r (threads 0..RING-1) along z,
+  // bottom plane column bx (threads RING..RING+BX-1) along y.  Otherwise
+  // thread t stages halo voxels t, t + WG, ...
+  constexpr int RING = G::RING;
+  const bool hcol = HRUNS && t < RING + G::BX;
+  const bool ring = t < RING;
+  int cbx = 0, cby = 0;
+  if (HRUNS) {
+    if (ring) {
+      if (t < G::BX) {
+        cbx = t;
+        cby = 0;
+      } else if (t < 2 * G::BX) {
+        cbx = t - G::BX;
+        cby = G::BY - 1;
+      } else {
+        cbx = ((t - 2 * G::BX) & 1) ? G::BX - 1 : 0;
+        cby = ((t - 2 * G::BX) >> 1) + 1;
+      }
+    } else {
+      cbx = t - RING;
+      cby = 0;
+    }
+  }
+  const int hlen = ring ? G::TZ : G::BY;
+  bool hok[HV];
+#pragma unroll
+  for (int k = 0; k < HV; k++) {
+    int bx, by, bz;
+    if (HRUNS) {
+      bx = cbx;
+      by = ring ? cby : k;
+      bz = ring ? k + 1 : 0;
+    } else {
+      G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
+    }
+    const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+    const bool live = HRUNS ? (hcol && k < hlen) : (t + WG * k < G::HALO);
+    hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
+    hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < PT; k++) col[k] = inb[k] ? col[k] : T(0);
+#pragma unroll
+  for (int k = 0; k < HV; k++) hv[k] = hok[k] ? hv[k] : T(0);
   {
     uint32_t start = 0;
 #pragma unroll
     for (int k = 0; k < PT; k++) {
       int vx, vy, vz;
       G::voxel(t, k, vx, vy, vz);
-      const int64_t gy = y0 + vy, gz = z0 + vz;
-      T v = T(0);
-      if (gx < sx && gy < sy && gz < sz) v = in[gx + sx * (gy + sy * gz)];
-      col[k] = v;
+      const T v = col[k];
       sv[b0 + k * CSTEP] = v;
       const uint32_t nk = (uint32_t)G::template node<QK>(vx, vy, vz);
       if (k > 0 && fg(v) && column_link<T, MODE, NB, QK>(v, col[k - 1], delta, gmin, gmax)) {
@@ -390,42 +478,19 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
   }
   if constexpr (HRUNS) {
-    // 3D halo as columns: ring position r (threads 0..RING-1) along z, bottom
-    // plane column bx (threads RING..RING+BX-1) along y.  A halo voxel linked
-    // to its predecessor inside the same neighbouring tile continues a run;
-    // its link then covers every other position of its piece, so only run
-    // starts are queued for step 2.
-    constexpr int RING = G::RING, HLEN = G::TZ > G::BY ? G::TZ : G::BY;
-    if (t < RING + G::BX) {
-      const bool ring = t < RING;
-      int cbx, cby;
-      if (ring) {
-        if (t < G::BX) {
-          cbx = t;
-          cby = 0;
-        } else if (t < 2 * G::BX) {
-          cbx = t - G::BX;
-          cby = G::BY - 1;
-        } else {
-          cbx = ((t - 2 * G::BX) & 1) ? G::BX - 1 : 0;
-          cby = ((t - 2 * G::BX) >> 1) + 1;
-        }
-      } else {
-        cbx = t - RING;
-        cby = 0;
-      }
-      const int len = ring ? G::TZ : G::BY;
+    // A halo voxel linked to its predecessor inside the same neighbouring
+    // tile continues a run; its link then covers every other position of its
+    // piece, so only run starts are queued for step 2.
+    if (hcol) {
       const int cpos = ring ? 0 : 1;  // E or K
       T prev = T(0);
       uint32_t start = 0;
 #pragma unroll
       for (int k = 0; k < HLEN; k++) {
-        if (k >= len) break;
+        if (k >= hlen) break;
         const int bx = cbx, by = ring ? cby : k, bz = ring ? k + 1 : 0;
         const int h = ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k;
-        const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = z0 + bz - 1;
-        T v = T(0);
-        if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
+        const T v = hv[k];
         sv[G::box(bx, by, bz)] = v;
         const bool linkable = ring ? k >= 1 : (k >= 2 && k <= G::TY);
         bool member = false;
@@ -440,14 +505,15 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       }
     }
   } else {
-    for (int h = t; h < G::HALO; h += WG) {
-      int bx, by, bz;
-      G::halo_coords(h, bx, by, bz);
-      const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-      T v = T(0);
-      if (hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz) v = in[hx + sx * (hy + sy * hz)];
-      sv[G::box(bx, by, bz)] = v;
-      par[G::TV + h] = G::TV + h;
+#pragma unroll
+    for (int k = 0; k < HV; k++) {
+      const int h = t + WG * k;
+      if (h < G::HALO) {
+        int bx, by, bz;
+        G::halo_coords(h, bx, by, bz);
+        sv[G::box(bx, by, bz)] = hv[k];
+        par[G::TV + h] = G::TV + h;
+      }
     }
   }
   __syncthreads();
