@@ -328,6 +328,13 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
   else return col_covers_all;
 }
 
+// Waves per SIMD the local kernel is compiled for: the LDS of one tile
+// (about 31 KB for 4-byte voxels) admits 5 workgroups of 4 waves per CU, so
+// registers are held to what 5 waves per SIMD allow.
+#ifndef LOCAL_WAVES_PER_EU
+#define LOCAL_WAVES_PER_EU 5
+#endif
+
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
 // several connectivities at run time allocate it once).
 template <class T, class G>
@@ -754,7 +761,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 }
 
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(WG) void k_local(LocalParams p) {
+__global__ __launch_bounds__(WG, LOCAL_WAVES_PER_EU) void k_local(LocalParams p) {
   __shared__ LocalLds<T, G> L;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
@@ -764,7 +771,7 @@ __global__ __launch_bounds__(WG) void k_local(LocalParams p) {
 // workgroup finds its volume and runs that volume's connectivity.  p carries
 // the shared per-tile arrays; its geometry fields are taken from the table.
 template <class T, int MODE, class G>
-__global__ __launch_bounds__(WG) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
+__global__ __launch_bounds__(WG, LOCAL_WAVES_PER_EU) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
                                                     int nvol) {
   __shared__ LocalLds<T, G> L;
   const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
