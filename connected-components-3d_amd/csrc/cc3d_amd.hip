@@ -184,10 +184,11 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
     const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
     for (uint32_t i = lane; i < nq; i += 64) {
       const uint32_t rq = p.REQ[slot * G::HALO + i];
-      const int ps = rq >> 16, h = rq & 0xFFFF;
-      const int lx = ps & (G::TX - 1), ly = (ps / G::TX) & (G::TY - 1), lz = ps / (G::TX * G::TY);
-      const uint32_t a = gkey<QK>(x0 + lx, y0 + ly, z0 + lz, v.sx, v.sy, v.osx) + v.key0;
+      const int rn = rq >> 16, h = rq & 0xFFFF;
       int bx, by, bz;
+      G::template nid_box<QK>(rn, bx, by, bz);  // the local root: a tile or a halo voxel
+      const uint32_t a =
+        gkey<QK>(x0 + bx - 1, y0 + by - 1, G::IS3D ? z0 + bz - 1 : 0, v.sx, v.sy, v.osx) + v.key0;
       G::halo_coords(h, bx, by, bz);
       const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
       const uint32_t b = lroot_key<G>(p, v, gx, gy, gz);

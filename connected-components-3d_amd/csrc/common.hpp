@@ -141,6 +141,37 @@ struct TileGeo {
   }
   // Position of a tile voxel inside its tile (raster order).
   __device__ static inline int pos(int lx, int ly, int lz) { return lx + TX * (ly + TY * lz); }
+  // Union-find node of a box position (tile or halo voxel).  Node order is
+  // global key order, so the root of every LDS tree is its member with the
+  // smallest key: the box raster index, or for the binary 2D 8-connected
+  // numbering the box-local 2x2 block key (x and y shifted by one so that
+  // block parity matches the global coordinates; tile origins are even).
+  static constexpr int BW = (BX + 2) / 2;
+  template <int QK>
+  static constexpr int NIDS = QK == Q_BLOCK8 ? 4 * BW * ((BY >> 1) + 1) : BOXN;
+  static constexpr int NIDS_MAX = NIDS<Q_BLOCK8> > BOXN ? NIDS<Q_BLOCK8> : BOXN;
+  template <int QK>
+  __device__ static inline int nid(int bx, int by, int bz) {
+    if (QK == Q_BLOCK8) {
+      const int xb = bx + 1, yb = by + 1;
+      return (((yb >> 1) * BW + (xb >> 1)) << 2) | ((yb & 1) << 1) | (xb & 1);
+    }
+    return box(bx, by, bz);
+  }
+  template <int QK>
+  __device__ static inline void nid_box(int n, int& bx, int& by, int& bz) {
+    if (QK == Q_BLOCK8) {
+      const int blk = n >> 2;
+      bx = (((blk % BW) << 1) | (n & 1)) - 1;
+      by = (((blk / BW) << 1) | ((n >> 1) & 1)) - 1;
+      bz = 0;
+      return;
+    }
+    bz = n / (BX * BY);
+    const int r = n - bz * (BX * BY);
+    by = r / BX;
+    bx = r - by * BX;
+  }
   // Thread t owns a column of PER_THREAD voxels along the column axis (z in
   // 3D, y in 2D); voxel k of the column.
   __device__ static inline void voxel(int t, int k, int& lx, int& ly, int& lz) {

@@ -140,9 +140,8 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
   auto val = [&](int i) -> T { return sv[b + box_off<G>(i)]; };
   auto nd = [&](int i) -> uint32_t {
     const Off o = NbSets<IS3D>::off(i);
-    if (HALO) return G::TV + G::halo_index(bx + o.dx, by + o.dy, bz + o.dz);
-    if (QK != Q_BLOCK8 && !bd.H) return pn + node_off<G>(i);
-    return G::template node_box<QK>(bx + o.dx, by + o.dy, bz + o.dz);
+    if (QK != Q_BLOCK8) return pn + box_off<G>(i);  // node ids are box indices
+    return G::template nid<QK>(bx + o.dx, by + o.dy, bz + o.dz);
   };
   auto cover = [&](int i) -> uint32_t { return HALO ? NH::adj(i) : cover_from<NB>(i, bd); };
   // node of position i for a run-time i (offsets unpacked from 2-bit fields)
@@ -151,9 +150,8 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
     const int dx = (int)((PX >> (2 * i)) & 3u) - 1;
     const int dy = (int)((PY >> (2 * i)) & 3u) - 1;
     const int dz = (int)((PZ >> (2 * i)) & 3u) - 1;
-    if (HALO) return G::TV + G::halo_index(bx + dx, by + dy, bz + dz);
-    if (QK != Q_BLOCK8 && !bd.H) return pn + dx + G::TX * (dy + G::TY * dz);
-    return G::template node_box<QK>(bx + dx, by + dy, bz + dz);
+    if (QK != Q_BLOCK8) return pn + dx + G::BX * (dy + G::BY * dz);
+    return G::template nid<QK>(bx + dx, by + dy, bz + dz);
   };
   auto ok = [&](int i) -> bool { return !HALO || ((elig >> i) & 1u); };
   uint32_t a = pn;
@@ -340,12 +338,14 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
 template <class T, class G>
 struct LocalLds {
   static constexpr int NW = WG / 64;
+  static constexpr int RW = (G::NIDS_MAX + 31) / 32;  // words of the root bitmask
   T sv[G::BOXN];
-  uint32_t par[G::NN];
+  uint32_t par[G::NIDS_MAX];
   uint16_t queue[G::TV];
   uint16_t hqueue[G::IS3D ? G::HALO : 1];
-  uint32_t wbase[G::PER_THREAD * NW + 1];
-  uint32_t qcount, nreq, hcount;
+  uint32_t rbits[RW];    // local roots that own tile voxels
+  uint32_t rprefix[RW];  // exclusive popcount prefix of rbits
+  uint32_t qcount, nreq, hcount, nroots;
   unsigned long long red_epl[NW], red_prov[NW];
   long long red_first[NW], red_last[NW];
 };
@@ -353,6 +353,14 @@ struct LocalLds {
 // Labels tile `tile` of the volume described by p.  Per-tile outputs go to
 // index `slot` of RL/J/REQ/TR/TQ/tcnt; keys are offset by `keybase` (both
 // are `tile` and 0 for a single volume; a batch concatenates volumes).
+//
+// Union-find nodes are box positions numbered in global key order (tile and
+// halo voxels alike), so every LDS tree is rooted at its member with the
+// smallest key.  A local component's root can then be a halo voxel: its key
+// stands for the component in the global forest, and the requests of its
+// halo pieces join it to the neighbouring tiles' forests.  The first voxel of
+// a column links to its predecessor (the halo voxel below in 3D, the voxel
+// above in 2D) with a plain store, like the rest of the column.
 template <class T, int MODE, int NB, int QK, class G>
 __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t slot, uint32_t keybase,
                                   LocalLds<T, G>& L) {
@@ -362,15 +370,17 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr bool PIECES = !(QK == Q_BIN4 || QK == Q_CONT8);
   constexpr bool IS3D = G::IS3D;
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
+  constexpr int RW = LocalLds<T, G>::RW;
   static_assert(HPT <= 32, "halo piece flags fit one word");
+  static_assert(RW <= 128, "one wave scans the root bitmask");
   // 3D transitive 26/18: halo staged as runs (see step 1)
   constexpr bool HRUNS = PIECES && IS3D && MODE != M_CONT && (NB == 26 || NB == 18);
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
 
   T* sv = L.sv;
   uint32_t* par = L.par;
   uint16_t* queue = L.queue;
   uint16_t* hqueue = L.hqueue;
-  uint32_t* wbase = L.wbase;
   uint32_t& qcount = L.qcount;
   uint32_t& nreq = L.nreq;
   uint32_t& hcount = L.hcount;
@@ -395,6 +405,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     nreq = 0;
     hcount = 0;
   }
+  for (int w = t; w < RW; w += WG) L.rbits[w] = 0;
   __syncthreads();
 
   // this thread's column
@@ -402,17 +413,23 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   G::voxel(t, 0, lx, ly0, lz0);
   const int64_t gx = x0 + lx;
   const int b0 = G::box(lx + 1, ly0 + 1, IS3D ? 1 : 0);
+  auto vnode = [&](int k) -> uint32_t {  // node of column voxel k
+    if (QK != Q_BLOCK8) return (uint32_t)(b0 + k * CSTEP);
+    int vx, vy, vz;
+    G::voxel(t, k, vx, vy, vz);
+    return (uint32_t)G::template nid<QK>(vx + 1, vy + 1, IS3D ? vz + 1 : 0);
+  };
 
   // 1. stage the column (registers + LDS) and the halo; voxels outside the
   //    volume read as background.  Column runs get their parent here.  All
-  //    global loads (column and halo) are issued before any value is used:
-  //    out-of-volume voxels load a valid address and are replaced by
-  //    background afterwards, so one wait covers every load.
+  //    global loads (column, its predecessor, halo) are issued before any
+  //    value is used: out-of-volume voxels load a valid address and are
+  //    replaced by background afterwards, so one wait covers every load.
   constexpr int HLEN = G::TZ > G::BY ? G::TZ : G::BY;
   constexpr int HV = HRUNS ? HLEN : HPT;
   T col[PT];
   T hv[HV];
-  uint32_t runs = 0;  // bit k: voxel k continues the run of voxel k-1
+  uint32_t runs = 0;  // bit k: voxel k continues the run of its predecessor
   bool inb[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++) {
@@ -422,6 +439,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     inb[k] = gx < sx && gy < sy && gz < sz;
     col[k] = in[inb[k] ? gx + sx * (gy + sy * gz) : 0];
   }
+  // predecessor of voxel 0: (x, y, z0 - 1) in 3D, (x, y - 1) in 2D
+  const int64_t py = IS3D ? y0 + ly0 : y0 + ly0 - 1, pz = IS3D ? z0 - 1 : 0;
+  const bool pok = gx < sx && py >= 0 && py < sy && pz >= 0 && pz < sz;
+  T pv = in[pok ? gx + sx * (py + sy * pz) : 0];
   // 3D halo as columns (HRUNS): ring position r (threads 0..RING-1) along z,
   // bottom plane column bx (threads RING..RING+BX-1) along y.  Otherwise
   // thread t stages halo voxels t, t + WG, ...
@@ -465,18 +486,18 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
 #pragma unroll
   for (int k = 0; k < PT; k++) col[k] = inb[k] ? col[k] : T(0);
+  pv = pok ? pv : T(0);
 #pragma unroll
   for (int k = 0; k < HV; k++) hv[k] = hok[k] ? hv[k] : T(0);
   {
-    uint32_t start = 0;
+    uint32_t start = QK != Q_BLOCK8 ? (uint32_t)(b0 - CSTEP)
+                                    : (uint32_t)G::template nid<QK>(lx + 1, ly0, 0);  // predecessor node
 #pragma unroll
     for (int k = 0; k < PT; k++) {
-      int vx, vy, vz;
-      G::voxel(t, k, vx, vy, vz);
       const T v = col[k];
       sv[b0 + k * CSTEP] = v;
-      const uint32_t nk = (uint32_t)G::template node<QK>(vx, vy, vz);
-      if (k > 0 && fg(v) && column_link<T, MODE, NB, QK>(v, col[k - 1], delta, gmin, gmax)) {
+      const uint32_t nk = vnode(k);
+      if (fg(v) && column_link<T, MODE, NB, QK>(v, k > 0 ? col[k - 1] : pv, delta, gmin, gmax)) {
         runs |= 1u << k;
       } else {
         start = nk;
@@ -497,16 +518,17 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         if (k >= hlen) break;
         const int bx = cbx, by = ring ? cby : k, bz = ring ? k + 1 : 0;
         const int h = ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k;
+        const int b = G::box(bx, by, bz);
         const T v = hv[k];
-        sv[G::box(bx, by, bz)] = v;
+        sv[b] = v;
         const bool linkable = ring ? k >= 1 : (k >= 2 && k <= G::TY);
         bool member = false;
         if (fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax)) {
           const uint32_t el = halo_elig<G>(bx, by, bz);
           member = ((Nbhd<NB>::adj(cpos) | (1u << cpos)) & el) == (el & ((1u << Nbhd<NB>::N) - 1u));
         }
-        if (!member) start = G::TV + h;
-        par[G::TV + h] = start;
+        if (!member) start = (uint32_t)b;
+        par[b] = start;
         if (fg(v) && !member) hqueue[atomicAdd(&hcount, 1u)] = (uint16_t)h;
         prev = v;
       }
@@ -519,7 +541,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         int bx, by, bz;
         G::halo_coords(h, bx, by, bz);
         sv[G::box(bx, by, bz)] = hv[k];
-        par[G::TV + h] = G::TV + h;
+        const int n = G::template nid<QK>(bx, by, bz);
+        par[n] = n;
       }
     }
   }
@@ -534,7 +557,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
       const int b = G::box(bx, by, bz);
-      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, G::TV + h, bx, by, bz, sv[b], Border{},
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{},
                                            halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
     }
   } else if constexpr (PIECES) {
@@ -544,8 +567,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       const int b = G::box(bx, by, bz);
       const T cur = sv[b];
       if (!fg(cur)) continue;
-      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, G::TV + h, bx, by, bz, cur, Border{}, halo_elig<G>(bx, by, bz),
-                                           false, delta, gmin, gmax, false);
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)G::template nid<QK>(bx, by, bz), bx, by, bz, cur,
+                                           Border{}, halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
     }
   }
 
@@ -556,10 +579,13 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   uint32_t needA = 0, needB = 0;
   {
     const bool xlo = lx == 0, xhi = lx == G::TX - 1;
-    const bool ylo_col = IS3D ? ly0 == 0 : false;  // 2D: only voxel 0 of the first column group
-    const bool yhi_col = IS3D ? ly0 == G::TY - 1 : false;
-    const Border bdc = make_border<IS3D>(xlo, xhi, ylo_col, yhi_col, false);
-    const bool covers_all = (cover_from<NB>(0, bdc) | 1u) == ((1u << Nbhd<NB>::N) - 1u);
+    // the column link of voxel 0 leaves the tile (3D: the z-1 halo plane; 2D:
+    // the halo row above for the first column group)
+    const Border bd0 = make_border<IS3D>(xlo, xhi, IS3D ? ly0 == 0 : ly0 == 0, IS3D && ly0 == G::TY - 1, IS3D);
+    const Border bdc = make_border<IS3D>(xlo, xhi, IS3D ? ly0 == 0 : false, IS3D && ly0 == G::TY - 1, false);
+    constexpr uint32_t ALLN = (1u << Nbhd<NB>::N) - 1u;
+    const bool covers_all0 = (cover_from<NB>(0, bd0) | 1u) == ALLN;
+    const bool covers_all = (cover_from<NB>(0, bdc) | 1u) == ALLN;
 #pragma unroll
     for (int k = 0; k < PT; k++) {
       const T cur = col[k];
@@ -571,7 +597,9 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       my_first = row < my_first ? row : my_first;
       my_last = row > my_last ? row : my_last;
       if ((runs >> k) & 1u) {
-        if (!column_done<T, MODE, NB, QK>(cur, col[k - 1], covers_all, delta, gmin, gmax)) needB |= 1u << k;
+        if (!column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0, delta,
+                                          gmin, gmax))
+          needB |= 1u << k;
       } else {
         needA |= 1u << k;
       }
@@ -609,7 +637,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     if (h < G::HALO) {
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
-      if (fg(sv[G::box(bx, by, bz)]) && (!PIECES || par[G::TV + h] == (uint32_t)(G::TV + h))) hpiece |= 1u << i;
+      const uint32_t n = (uint32_t)G::template nid<QK>(bx, by, bz);
+      if (fg(sv[G::box(bx, by, bz)]) && (!PIECES || par[n] == n)) hpiece |= 1u << i;
     }
   }
   const uint32_t nA = qcount & 0xFFFFu, nQ = nA + (qcount >> 16);
@@ -625,77 +654,80 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     const int b = G::box(bx, by, bz);
     const T cur = sv[b];
     const Border bd = make_border<IS3D>(vx == 0, vx == G::TX - 1, vy == 0, IS3D && vy == G::TY - 1, IS3D && vz == 0);
-    const uint32_t pn = (uint32_t)G::template node<QK>(vx, vy, vz);
+    const uint32_t pn = (uint32_t)G::template nid<QK>(bx, by, bz);
     const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, 0u, colq, delta,
                                                           gmin, gmax, y0 + vy == 0);
     if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
   }
   __syncthreads();
 
-  // 5. resolve roots: run members share their run start's root
+  // 5. resolve roots (run members after the first share their run start's
+  //    root) and mark the roots that own tile voxels
   uint32_t root[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++) {
-    int vx, vy, vz;
-    G::voxel(t, k, vx, vy, vz);
-    if (!fg(col[k])) root[k] = 0xFFFFFFFFu;
-    else if ((runs >> k) & 1u) root[k] = root[k > 0 ? k - 1 : 0];
-    else root[k] = lfind(par, (uint32_t)G::template node<QK>(vx, vy, vz));
+    if (!fg(col[k])) root[k] = NONE;
+    else if (k > 0 && ((runs >> k) & 1u)) root[k] = root[k - 1];
+    else root[k] = lfind(par, vnode(k));
   }
   uint32_t hroot[HPT];
 #pragma unroll
   for (int i = 0; i < HPT; i++) {
     const int h = t + WG * i;
-    hroot[i] = ((hpiece >> i) & 1u) ? lfind(par, G::TV + h) : 0xFFFFFFFFu;
+    uint32_t r = NONE;
+    if ((hpiece >> i) & 1u) {
+      int bx, by, bz;
+      G::halo_coords(h, bx, by, bz);
+      r = lfind(par, (uint32_t)G::template nid<QK>(bx, by, bz));
+    }
+    hroot[i] = r;
   }
-
-  // 6. rank local roots (ballot order: slice k, then wave, then lane)
-  bool isroot[PT];
-  uint64_t bal[PT];
 #pragma unroll
-  for (int k = 0; k < PT; k++) {
-    int vx, vy, vz;
-    G::voxel(t, k, vx, vy, vz);
-    isroot[k] = root[k] == (uint32_t)G::template node<QK>(vx, vy, vz);
-    bal[k] = __ballot(isroot[k]);
-    if (lane == 0) wbase[k * NW + wv] = __popcll(bal[k]);
-  }
+  for (int k = 0; k < PT; k++)
+    if (root[k] != NONE && (k == 0 || root[k] != root[k - 1])) atomicOr(&L.rbits[root[k] >> 5], 1u << (root[k] & 31));
   __syncthreads();
+
+  // 6. rank the marked roots in node (= key) order: one wave scans the bitmask
   if (wv == 0) {
-    static_assert(PT * NW <= 64, "one wave scans the per-wave root counts");
-    const uint32_t c = lane < PT * NW ? wbase[lane] : 0u;
-    uint32_t incl = c;
+    const int w0 = 2 * lane;
+    const uint32_t c0 = w0 < RW ? (uint32_t)__popc(L.rbits[w0]) : 0u;
+    const uint32_t c1 = w0 + 1 < RW ? (uint32_t)__popc(L.rbits[w0 + 1]) : 0u;
+    uint32_t incl = c0 + c1;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(incl, o);
       if (lane >= o) incl += y;
     }
-    if (lane < PT * NW) wbase[lane] = incl - c;
-    if (lane == 63) wbase[PT * NW] = incl;
+    const uint32_t ex = incl - c0 - c1;
+    if (w0 < RW) L.rprefix[w0] = ex;
+    if (w0 + 1 < RW) L.rprefix[w0 + 1] = ex + c0;
+    if (lane == 63) L.nroots = incl;
   }
   __syncthreads();
-  const uint32_t R = wbase[PT * NW];
-  const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t R = L.nroots;
+  auto rank = [&](uint32_t r) -> uint32_t {
+    return L.rprefix[r >> 5] + (uint32_t)__popc(L.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
+  };
   uint32_t* RLt = p.RL + slot * G::TV;
-#pragma unroll
-  for (int k = 0; k < PT; k++) {
-    if (isroot[k]) {
-      const uint32_t rk = wbase[k * NW + wv] + __popcll(bal[k] & ltmask);
-      par[root[k]] = rk;
-      int vx, vy, vz;
-      G::voxel(t, k, vx, vy, vz);
-      const uint32_t key = gkey<QK>(x0 + vx, y0 + vy, z0 + vz, sx, sy, p.osx) + keybase;
-      RLt[rk] = key;
+  for (int w = t; w < RW; w += WG) {
+    uint32_t bits = L.rbits[w];
+    uint32_t rk = L.rprefix[w];
+    while (bits) {
+      const int bpos = __builtin_ctz(bits);
+      bits &= bits - 1u;
+      int bx, by, bz;
+      G::template nid_box<QK>(32 * w + bpos, bx, by, bz);
+      const uint32_t key = gkey<QK>(x0 + bx - 1, y0 + by - 1, IS3D ? z0 + bz - 1 : 0, sx, sy, p.osx) + keybase;
+      RLt[rk++] = key;
       p.GP[key] = key;
     }
   }
-  __syncthreads();
 
   // 7. per-voxel root index, thread-major: one 8- or 16-byte store per thread
   static_assert(PT == 8, "J stores assume 8 voxels per thread");
   uint32_t jv[PT];
 #pragma unroll
-  for (int k = 0; k < PT; k++) jv[k] = root[k] == 0xFFFFFFFFu ? 0u : par[root[k]] + 1u;
+  for (int k = 0; k < PT; k++) jv[k] = root[k] == NONE ? 0u : rank(root[k]) + 1u;
   if (R <= 255) {
     uint64_t w = 0;
 #pragma unroll
@@ -710,17 +742,15 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     reinterpret_cast<uint4*>(p.J16 + slot * G::TV)[t] = w;
   }
 
-  // 8. cross-tile requests: one per halo piece attached to a local component
+  // 8. cross-tile requests: one per halo piece whose tree owns tile voxels,
+  //    as (root node, halo index)
   uint32_t* REQt = p.REQ + slot * G::HALO;
 #pragma unroll
   for (int i = 0; i < HPT; i++) {
-    const int h = t + WG * i;
     const uint32_t r = hroot[i];
-    if (r < (uint32_t)G::TV) {
-      int vx, vy, vz;
-      G::template node_to_local<QK>(r, vx, vy, vz);
-      const uint32_t slot = atomicAdd(&nreq, 1u);
-      REQt[slot] = ((uint32_t)G::pos(vx, vy, vz) << 16) | (uint32_t)h;
+    if (r != NONE && ((L.rbits[r >> 5] >> (r & 31)) & 1u)) {
+      const uint32_t q = atomicAdd(&nreq, 1u);
+      REQt[q] = (r << 16) | (uint32_t)(t + WG * i);
     }
   }
 
@@ -741,11 +771,11 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
   __syncthreads();
   if (t == 0) {
-    unsigned long long e = 0, pv = 0;
+    unsigned long long e = 0, pv2 = 0;
     long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
     for (int w = 0; w < NW; w++) {
       e += L.red_epl[w];
-      pv += L.red_prov[w];
+      pv2 += L.red_prov[w];
       f = L.red_first[w] < f ? L.red_first[w] : f;
       l = L.red_last[w] > l ? L.red_last[w] : l;
     }
@@ -753,7 +783,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     p.TQ[slot] = nreq;
     TileCnt c;
     c.epl = (uint32_t)e;
-    c.prov = (uint32_t)pv;
+    c.prov = (uint32_t)pv2;
     c.first_row = f;
     c.last_row = l;
     p.tcnt[slot] = c;
