@@ -509,6 +509,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     // A halo voxel linked to its predecessor inside the same neighbouring
     // tile continues a run; its link then covers every other position of its
     // piece, so only run starts are queued for step 2.
+    uint32_t hneed = 0;  // bit k: halo voxel k of this column is a run start
     if (hcol) {
       const int cpos = ring ? 0 : 1;  // E or K
       T prev = T(0);
@@ -517,7 +518,6 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       for (int k = 0; k < HLEN; k++) {
         if (k >= hlen) break;
         const int bx = cbx, by = ring ? cby : k, bz = ring ? k + 1 : 0;
-        const int h = ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k;
         const int b = G::box(bx, by, bz);
         const T v = hv[k];
         sv[b] = v;
@@ -529,9 +529,27 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         }
         if (!member) start = (uint32_t)b;
         par[b] = start;
-        if (fg(v) && !member) hqueue[atomicAdd(&hcount, 1u)] = (uint16_t)h;
+        if (fg(v) && !member) hneed |= 1u << k;
         prev = v;
       }
+    }
+    // queue the run starts: a wave-level scan and one LDS atomic per wave
+    const uint32_t hc = (uint32_t)__popc(hneed);
+    uint32_t hincl = hc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(hincl, o);
+      if (lane >= o) hincl += y;
+    }
+    const uint32_t htot = __shfl(hincl, 63);
+    if (htot) {
+      uint32_t hbase = 0;
+      if (lane == 63) hbase = atomicAdd(&hcount, htot);
+      hbase = __shfl(hbase, 63) + hincl - hc;
+#pragma unroll
+      for (int k = 0; k < HLEN; k++)
+        if ((hneed >> k) & 1u)
+          hqueue[hbase++] = (uint16_t)(ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k);
     }
   } else {
 #pragma unroll
@@ -683,8 +701,12 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     hroot[i] = r;
   }
 #pragma unroll
-  for (int k = 0; k < PT; k++)
-    if (root[k] != NONE && (k == 0 || root[k] != root[k - 1])) atomicOr(&L.rbits[root[k] >> 5], 1u << (root[k] & 31));
+  for (int k = 0; k < PT; k++) {
+    const uint32_t r = root[k];
+    // most columns of a component share its root: skip the atomic once set
+    if (r != NONE && (k == 0 || r != root[k - 1]) && !((lload(&L.rbits[r >> 5]) >> (r & 31)) & 1u))
+      atomicOr(&L.rbits[r >> 5], 1u << (r & 31));
+  }
   __syncthreads();
 
   // 6. rank the marked roots in node (= key) order: one wave scans the bitmask
@@ -727,7 +749,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   static_assert(PT == 8, "J stores assume 8 voxels per thread");
   uint32_t jv[PT];
 #pragma unroll
-  for (int k = 0; k < PT; k++) jv[k] = root[k] == NONE ? 0u : rank(root[k]) + 1u;
+  for (int k = 0; k < PT; k++)
+    jv[k] = root[k] == NONE ? 0u : (k > 0 && root[k] == root[k - 1]) ? jv[k > 0 ? k - 1 : 0] : rank(root[k]) + 1u;
   if (R <= 255) {
     uint64_t w = 0;
 #pragma unroll
