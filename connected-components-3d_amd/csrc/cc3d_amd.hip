@@ -167,13 +167,18 @@ __device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int
   return p.RL[slot * G::TV + j - 1];
 }
 
-// Cross-tile unions: one wave per tile walks that tile's requests.
+// Lanes that share one tile in the per-tile global kernels: a tile holds a few
+// tens of requests or local roots, so a whole wave per tile would leave most
+// lanes idle.
+constexpr int TILE_LANES = 16;
+
+// Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests.
 template <class G, int QK>
 __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
-  for (int64_t slot = wave; slot < p.ntiles; slot += nwaves) {
+  const int lane = threadIdx.x & 63, sub = lane % TILE_LANES;
+  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
+  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
+  for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
     const uint32_t nq = p.TQ[slot];
     if (nq == 0) continue;
     const VolGeo v = vol_geo(p, slot);
@@ -182,7 +187,7 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
     const int64_t tr = tile - tz * v.ntx * v.nty;
     const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
     const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
-    for (uint32_t i = lane; i < nq; i += 64) {
+    for (uint32_t i = sub; i < nq; i += TILE_LANES) {
       const uint32_t rq = p.REQ[slot * G::HALO + i];
       const int rn = rq >> 16, h = rq & 0xFFFF;
       int bx, by, bz;
@@ -307,12 +312,12 @@ __global__ __launch_bounds__(WG) void k_periodic(GlobalParams p, const T* in, in
 // Global roots: compress every local root onto its root; roots mark their key.
 template <class G>
 __global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
-  for (int64_t tile = wave; tile < p.ntiles; tile += nwaves) {
+  const int sub = threadIdx.x % TILE_LANES;
+  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
+  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
+  for (int64_t tile = group; tile < p.ntiles; tile += ngroups) {
     const uint32_t R = p.TR[tile];
-    for (uint32_t j = lane; j < R; j += 64) {
+    for (uint32_t j = sub; j < R; j += TILE_LANES) {
       const uint32_t g = p.RL[tile * G::TV + j];
       const uint32_t r = gfind(p.GP, g);
       if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
@@ -395,14 +400,14 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
 // its global root's key among the roots of its volume.
 template <class G>
 __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * WG) >> 6;
-  for (int64_t slot = wave; slot < p.ntiles; slot += nwaves) {
+  const int sub = threadIdx.x % TILE_LANES;
+  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
+  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
+  for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
     const uint32_t R = p.TR[slot];
     if (R == 0) continue;
     const uint32_t base = p.nvol ? p.prefix[vol_geo(p, slot).key0 >> 5] : 0u;
-    for (uint32_t j = lane; j < R; j += 64) {
+    for (uint32_t j = sub; j < R; j += TILE_LANES) {
       const uint32_t g = p.RL[slot * G::TV + j];
       const uint32_t r = p.GP[g];
       const uint32_t wd = r >> 5;
@@ -961,7 +966,7 @@ template <class G>
 int run_global(cc3d_amd_job* j) {
   const GlobalParams& p = j->gp;
   hipStream_t s = j->stream;
-  const unsigned wave_blocks = grid_for(j->g.ntiles, 4, 65536);
+  const unsigned wave_blocks = grid_for(j->g.ntiles, WG / TILE_LANES, 65536);
   {
   ProfScope ps_("k_requests", s);
   if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
@@ -1634,7 +1639,7 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     hipLaunchKernelGGL(k_tile_counters_batch, dim3(b->n), dim3(WG), 0, s, b->lp.tcnt, p.vtile0, p.ctr + 1);
   }
   HIPCHECK(hipGetLastError());
-  const unsigned wave_blocks = grid_for(b->ntiles, 4, 65536);
+  const unsigned wave_blocks = grid_for(b->ntiles, WG / TILE_LANES, 65536);
   {
     ProfScope ps_("k_requests", s);
     hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
