@@ -15,6 +15,8 @@ Prints one JSON line.  Also reports:
   roofline      the dominant kernel's algorithmic bytes / its mean duration,
                 measured with HIP events on the launch stream during the timed
                 steps; traffic from the committed rocprofv3 counter summary
+  kernels       every kernel's mean duration, from one extra untimed pass with
+                events around each launch
   path_roofline the whole call: 8 bytes/voxel (4 in + 4 out) / step time
   cpu_baseline  the reference implementation (oracle/_ref, compiled from the
                 reference sources) timed on one host core on the same volume,
@@ -309,8 +311,16 @@ def main() -> int:
   lib.cc3d_amd_profile_collect.restype = ctypes.c_int
   lib.cc3d_amd_profile_collect.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+  # per-kernel breakdown from one untimed profiled pass of the same steps
   _profile_rows(lib)  # drop anything recorded before
   lib.cc3d_amd_profile_enable(1)
+  for _ in range(min(args.steps, 5)):
+    step()
+  torch.cuda.synchronize()
+  lib.cc3d_amd_profile_enable(0)
+  rows_all = _profile_rows(lib)
+  # timed steps; HIP events on the launch stream around the dominant kernel only
+  lib.cc3d_amd_profile_enable(2)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   for _ in range(args.steps):
@@ -318,7 +328,8 @@ def main() -> int:
   torch.cuda.synchronize()
   elapsed = time.perf_counter() - t0
   lib.cc3d_amd_profile_enable(0)
-  rows = _profile_rows(lib)
+  rows = dict(rows_all)
+  rows.update(_profile_rows(lib))  # k_local: measured over the timed steps
   _progress(f"headline: {args.steps} steps in {elapsed * 1e3:.1f} ms")
 
   ms_per_step = elapsed * 1e3 / args.steps

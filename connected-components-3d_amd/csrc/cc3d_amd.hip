@@ -796,7 +796,7 @@ struct ProfRec {
 };
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<hipEvent_t> g_prof_pool;
-static bool g_prof_on = false;
+static int g_prof_on = 0;  // 0 off, 1 every kernel, 2 the local kernel only
 static thread_local bool g_prof_suspended = false;  // inside a graph capture
 static std::mutex g_prof_mu;
 
@@ -817,6 +817,7 @@ struct ProfScope {
   const char* name;
   ProfScope(const char* n, hipStream_t st) : s(st), name(n) {
     if (!g_prof_on || g_prof_suspended) return;
+    if (g_prof_on == 2 && std::strcmp(n, "k_local") != 0) return;  // dominant kernel only
     std::lock_guard<std::mutex> g(g_prof_mu);
     a = prof_event();
     (void)hipEventRecord(a, s);
@@ -831,6 +832,32 @@ struct ProfScope {
 };
 
 static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Pinned host buffers for the jobs' counter read-back, reused across calls
+// (pinned allocation and release cost far more than a small labelling call).
+static std::mutex g_pin_mu;
+static std::vector<Counters*> g_pin_pool;
+
+static Counters* pinned_counters() {
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    if (!g_pin_pool.empty()) {
+      Counters* c = g_pin_pool.back();
+      g_pin_pool.pop_back();
+      return c;
+    }
+  }
+  Counters* c = nullptr;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c), sizeof(Counters), hipHostMallocDefault) != hipSuccess) return nullptr;
+  return c;
+}
+
+static void release_pinned_counters(Counters* c) {
+  if (!c) return;
+  std::lock_guard<std::mutex> g(g_pin_mu);
+  if (g_pin_pool.size() < 64) g_pin_pool.push_back(c);
+  else (void)hipHostFree(c);
+}
 static inline unsigned grid_for(int64_t items, int64_t per_block, int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, cap));
 }
@@ -1133,7 +1160,8 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
     j->ws = static_cast<char*>(p);
     j->own_ws = true;
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&j->h_ctr), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+  j->h_ctr = pinned_counters();
+  if (!j->h_ctr) {
     cc3d_amd_job_free(j);
     return fail(CC3D_AMD_ERR_NOMEM, "could not allocate pinned counters");
   }
@@ -1300,7 +1328,7 @@ int cc3d_amd_label_finish(cc3d_amd_job* j, void* out, int out_dtype) {
 void cc3d_amd_job_free(cc3d_amd_job* j) {
   if (!j) return;
   if (j->own_ws && j->ws) (void)hipFreeAsync(j->ws, j->stream);
-  if (j->h_ctr) (void)hipHostFree(j->h_ctr);
+  release_pinned_counters(j->h_ctr);
   delete j;
 }
 
@@ -1402,7 +1430,7 @@ int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, in
 
 void cc3d_amd_profile_enable(int on) {
   std::lock_guard<std::mutex> g(g_prof_mu);
-  g_prof_on = on != 0;
+  g_prof_on = on == 2 ? 2 : (on != 0 ? 1 : 0);
 }
 
 // Waits for the recorded events and returns up to cap (name, total ms, count)

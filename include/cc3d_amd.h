@@ -169,8 +169,9 @@ int cc3d_amd_dust_apply(void* img, int img_dtype, const void* labels, int label_
 const char* cc3d_amd_follow_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the launch stream.  No
- * reference counterpart (measurement support).  collect() waits for the
- * events and returns rows (name, total ms, launches) aggregated by kernel. */
+ * reference counterpart (measurement support).  on: 0 off, 1 every kernel,
+ * 2 only the local labelling kernel.  collect() waits for the events and
+ * returns rows (name, total ms, launches) aggregated by kernel. */
 void cc3d_amd_profile_enable(int on);
 int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* counts, int cap);
 
