@@ -371,7 +371,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr bool IS3D = G::IS3D;
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
   constexpr int RW = LocalLds<T, G>::RW;
-  static_assert(HPT <= 32, "halo piece flags fit one word");
+  static_assert(HPT <= 32 && (!G::IS3D || (G::TZ <= 32 && G::BY <= 32)), "halo piece flags fit one word");
   static_assert(RW <= 128, "one wave scans the root bitmask");
   // 3D transitive 26/18: halo staged as runs (see step 1)
   constexpr bool HRUNS = PIECES && IS3D && MODE != M_CONT && (NB == 26 || NB == 18);
@@ -471,18 +471,25 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   bool hok[HV];
 #pragma unroll
   for (int k = 0; k < HV; k++) {
-    int bx, by, bz;
-    if (HRUNS) {
-      bx = cbx;
-      by = ring ? cby : k;
-      bz = ring ? k + 1 : 0;
-    } else {
-      G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
+    hok[k] = false;
+    hv[k] = T(0);
+  }
+  if (!HRUNS || hcol) {  // waves holding no halo column skip the address work
+#pragma unroll
+    for (int k = 0; k < HV; k++) {
+      int bx, by, bz;
+      if (HRUNS) {
+        bx = cbx;
+        by = ring ? cby : k;
+        bz = ring ? k + 1 : 0;
+      } else {
+        G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
+      }
+      const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+      const bool live = HRUNS ? (k < hlen) : (t + WG * k < G::HALO);
+      hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
+      hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
     }
-    const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-    const bool live = HRUNS ? (hcol && k < hlen) : (t + WG * k < G::HALO);
-    hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
-    hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
   }
 #pragma unroll
   for (int k = 0; k < PT; k++) col[k] = inb[k] ? col[k] : T(0);
@@ -592,8 +599,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 
   // 3. counters over the column and the work queue: voxels still needing
   //    neighbour unions (run starts first, run members from the back)
-  unsigned long long my_epl = 0, my_prov = 0;
-  long long my_first = 0x7FFFFFFFFFFFFFFFll, my_last = -1;
+  // per-thread counts fit 32 bits; rows are tile-local (ly + TY * lz), whose
+  // order is the order of the global rows y + sy * z
+  uint32_t my_epl = 0, my_prov = 0;
+  int my_first = 0x7FFFFFFF, my_last = -1;
   uint32_t needA = 0, needB = 0;
   {
     const bool xlo = lx == 0, xhi = lx == G::TX - 1;
@@ -611,7 +620,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       int vx, vy, vz;
       G::voxel(t, k, vx, vy, vz);
       if (cur != sv[b0 + k * CSTEP - 1]) my_epl++;
-      const long long row = (y0 + vy) + sy * (z0 + vz);
+      const int row = vy + G::TY * vz;
       my_first = row < my_first ? row : my_first;
       my_last = row > my_last ? row : my_last;
       if ((runs >> k) & 1u) {
@@ -647,16 +656,24 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   __syncthreads();
 
   // halo pieces are the roots left by step 2 (without pieces every
-  // foreground halo voxel stands for itself)
+  // foreground halo voxel stands for itself).  Entry i of this thread: halo
+  // column voxel i under HRUNS, else halo voxel t + WG * i.
+  auto hnode = [&](int i) -> uint32_t {
+    if (HRUNS) return (uint32_t)G::box(cbx, ring ? cby : i, ring ? i + 1 : 0);
+    int bx, by, bz;
+    G::halo_coords(t + WG * i, bx, by, bz);
+    return (uint32_t)G::template nid<QK>(bx, by, bz);
+  };
+  auto hindex = [&](int i) -> uint32_t {
+    if (HRUNS) return (uint32_t)(ring ? G::BX * G::BY + i * RING + t : cbx + G::BX * i);
+    return (uint32_t)(t + WG * i);
+  };
   uint32_t hpiece = 0;
 #pragma unroll
-  for (int i = 0; i < HPT; i++) {
-    const int h = t + WG * i;
-    if (h < G::HALO) {
-      int bx, by, bz;
-      G::halo_coords(h, bx, by, bz);
-      const uint32_t n = (uint32_t)G::template nid<QK>(bx, by, bz);
-      if (fg(sv[G::box(bx, by, bz)]) && (!PIECES || par[n] == n)) hpiece |= 1u << i;
+  for (int i = 0; i < HV; i++) {
+    if (fg(hv[i])) {  // hv is background outside the halo and the volume
+      const uint32_t n = hnode(i);
+      if (!PIECES || par[n] == n) hpiece |= 1u << i;
     }
   }
   const uint32_t nA = qcount & 0xFFFFu, nQ = nA + (qcount >> 16);
@@ -688,18 +705,9 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     else if (k > 0 && ((runs >> k) & 1u)) root[k] = root[k - 1];
     else root[k] = lfind(par, vnode(k));
   }
-  uint32_t hroot[HPT];
+  uint32_t hroot[HV];
 #pragma unroll
-  for (int i = 0; i < HPT; i++) {
-    const int h = t + WG * i;
-    uint32_t r = NONE;
-    if ((hpiece >> i) & 1u) {
-      int bx, by, bz;
-      G::halo_coords(h, bx, by, bz);
-      r = lfind(par, (uint32_t)G::template nid<QK>(bx, by, bz));
-    }
-    hroot[i] = r;
-  }
+  for (int i = 0; i < HV; i++) hroot[i] = ((hpiece >> i) & 1u) ? lfind(par, hnode(i)) : NONE;
 #pragma unroll
   for (int k = 0; k < PT; k++) {
     const uint32_t r = root[k];
@@ -769,11 +777,11 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   //    as (root node, halo index)
   uint32_t* REQt = p.REQ + slot * G::HALO;
 #pragma unroll
-  for (int i = 0; i < HPT; i++) {
+  for (int i = 0; i < HV; i++) {
     const uint32_t r = hroot[i];
     if (r != NONE && ((L.rbits[r >> 5] >> (r & 31)) & 1u)) {
       const uint32_t q = atomicAdd(&nreq, 1u);
-      REQt[q] = (r << 16) | (uint32_t)(t + WG * i);
+      REQt[q] = (r << 16) | hindex(i);
     }
   }
 
@@ -782,7 +790,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   for (int o = 32; o > 0; o >>= 1) {
     my_epl += __shfl_xor(my_epl, o);
     my_prov += __shfl_xor(my_prov, o);
-    const long long f = __shfl_xor(my_first, o), l = __shfl_xor(my_last, o);
+    const int f = __shfl_xor(my_first, o), l = __shfl_xor(my_last, o);
     my_first = f < my_first ? f : my_first;
     my_last = l > my_last ? l : my_last;
   }
@@ -796,11 +804,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   if (t == 0) {
     unsigned long long e = 0, pv2 = 0;
     long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
+    int fl = 0x7FFFFFFF, ll = -1;
     for (int w = 0; w < NW; w++) {
       e += L.red_epl[w];
       pv2 += L.red_prov[w];
-      f = L.red_first[w] < f ? L.red_first[w] : f;
-      l = L.red_last[w] > l ? L.red_last[w] : l;
+      fl = (int)L.red_first[w] < fl ? (int)L.red_first[w] : fl;
+      ll = (int)L.red_last[w] > ll ? (int)L.red_last[w] : ll;
+    }
+    if (ll >= 0) {  // tile-local rows to global rows
+      f = (y0 + fl % G::TY) + sy * (IS3D ? z0 + fl / G::TY : 0);
+      l = (y0 + ll % G::TY) + sy * (IS3D ? z0 + ll / G::TY : 0);
     }
     p.TR[slot] = R;
     p.TQ[slot] = nreq;
