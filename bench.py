@@ -259,6 +259,57 @@ def _served_runs(srv, cc3d, draw, burst, vox, nreq, loads, gen_s):
   }
 
 
+def configs_leg(torch, cc3d, synth, dev, steps=5):
+  """The other workloads at full size: GPU throughput with the input resident
+  in HBM, the reference on one host core, and a bit-exact check of the two."""
+  import oracle
+  cases = []
+  for p in (0.1, 0.3, 0.5):
+    for conn in (6, 26):
+      cases.append((f"512^3 uint8 Bernoulli p={p} binary {conn}-conn",
+                    lambda p=p, conn=conn: synth.bernoulli((512, 512, 512), p, seed=int(p * 10) * 100 + conn),
+                    dict(connectivity=conn, binary_image=True, out_dtype=np.uint32)))
+  cases.append(("16384^2 uint8 Bernoulli p=0.5 binary 8-conn",
+                lambda: synth.bernoulli((16384, 16384), 0.5, seed=16384),
+                dict(connectivity=8, binary_image=True, out_dtype=np.uint32)))
+  u64 = {}
+  for conn in (6, 18, 26):
+    def gen_u64():
+      if "v" not in u64:
+        u64["v"] = synth.voronoi((512, 512, 512), 40000, seed=64, id_range=(1, None), dtype=np.uint64)
+      return u64["v"]
+    cases.append((f"512^3 uint64 Voronoi 40k seeds {conn}-conn", gen_u64, dict(connectivity=conn, out_dtype=np.uint32)))
+  cases.append(("512^3 float32 tones, continuous delta=20 26-conn", lambda: synth.tones((512, 512, 512), 5000, seed=32),
+                dict(connectivity=26, delta=20, out_dtype=np.uint32)))
+  res = []
+  for name, gen, kw in cases:
+    vol = gen()
+    V = vol.size
+    if vol.ndim == 3:
+      d = torch.from_numpy(np.ravel(vol, order="F")).to(dev).as_strided(vol.shape, (1, vol.shape[0], vol.shape[0] * vol.shape[1]))
+    else:
+      d = torch.from_numpy(np.ravel(vol, order="F")).to(dev).as_strided(vol.shape, (1, vol.shape[0]))
+    out, N = cc3d.connected_components(d, return_N=True, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      out, N = cc3d.connected_components(d, return_N=True, **kw)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / steps
+    row = {"workload": name, "voxels": V, "gvox_s": V / gpu_s / 1e9, "ms": gpu_s * 1e3, "components": int(N),
+           "path_gbs": (vol.itemsize + 4) * V / gpu_s / 1e9}
+    if oracle.have_ref():
+      t0 = time.perf_counter()
+      ref, rN = oracle.connected_components(vol, return_N=True, backend="ref", **kw)
+      cpu_s = time.perf_counter() - t0
+      row["cpu_reference_gvox_s"] = V / cpu_s / 1e9
+      row["bit_exact_vs_reference"] = bool(rN == N and np.array_equal(out.cpu().numpy(), ref))
+    _progress(f"configs: {name}: {row['gvox_s']:.1f} Gvox/s")
+    res.append(row)
+    del d, out
+  return res
+
+
 def main() -> int:
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -267,6 +318,8 @@ def main() -> int:
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
   ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
   ap.add_argument("--no-served", action="store_true", help="skip the served leg")
+  ap.add_argument("--configs", action="store_true",
+                  help="also time the other workloads at full size against the reference")
   ap.add_argument("--calib", action="store_true",
                   help="also run one 256 MiB device copy (counter calibration for profiles)")
   args = ap.parse_args()
@@ -383,6 +436,10 @@ def main() -> int:
   if not args.no_batched:
     batched = batched_leg(torch, cc3d, synth, dev, cpu=not args.no_cpu)
 
+  configs = None
+  if args.configs:
+    configs = configs_leg(torch, cc3d, synth, dev)
+
   served = None
   if not args.no_served:
     torch.cuda.empty_cache()
@@ -411,6 +468,7 @@ def main() -> int:
     "bit_exact_vs_reference": exact,
     "batched": batched,
     "served": served,
+    "configs": configs,
     "generate_s": gen_s,
   }
   print(json.dumps(line))
