@@ -248,11 +248,14 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
     uint32_t conn = col ? 1u : 0u;
     lk |= conn;
     uint32_t front = conn;
+    uint32_t cv[NH::N];  // cover masks, computed once for the growth rounds
+#pragma unroll
+    for (int j = 0; j < NH::N; j++) cv[j] = cover(j);
     while (true) {
       while (front) {
         uint32_t nb = 0;
 #pragma unroll
-        for (int j = 0; j < NH::N; j++) nb |= ((front >> j) & 1u) ? cover(j) : 0u;
+        for (int j = 0; j < NH::N; j++) nb |= ((front >> j) & 1u) ? cv[j] : 0u;
         front = nb & lk & ~conn;
         conn |= front;
       }
@@ -431,13 +434,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   T hv[HV];
   uint32_t runs = 0;  // bit k: voxel k continues the run of its predecessor
   bool inb[PT];
+  // one base address per column, stepped by a stride that is the same for
+  // every thread; the bounds along the column are per tile
+  const int64_t cstride = IS3D ? sx * sy : sx;                   // voxel k -> k + 1
+  const int64_t ccount = IS3D ? sz - z0 : sy - (y0 + ly0);       // voxels left in the volume
+  const bool colok = gx < sx && y0 + ly0 < sy;  // the column's first voxel is in the volume
+  const T* cbase = in + (colok ? gx + sx * ((y0 + ly0) + sy * z0) : 0);
 #pragma unroll
   for (int k = 0; k < PT; k++) {
-    int vx, vy, vz;
-    G::voxel(t, k, vx, vy, vz);
-    const int64_t gy = y0 + vy, gz = z0 + vz;
-    inb[k] = gx < sx && gy < sy && gz < sz;
-    col[k] = in[inb[k] ? gx + sx * (gy + sy * gz) : 0];
+    inb[k] = colok && k < ccount;
+    col[k] = *(inb[k] ? cbase + k * cstride : in);
   }
   // predecessor of voxel 0: (x, y, z0 - 1) in 3D, (x, y - 1) in 2D
   const int64_t py = IS3D ? y0 + ly0 : y0 + ly0 - 1, pz = IS3D ? z0 - 1 : 0;
@@ -474,19 +480,31 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     hok[k] = false;
     hv[k] = T(0);
   }
-  if (!HRUNS || hcol) {  // waves holding no halo column skip the address work
+  if constexpr (HRUNS) {
+    if (hcol) {  // waves holding no halo column skip the address work
+      // ring column: (hx, hy) fixed, z = z0 .. z0 + TZ - 1; bottom column:
+      // (hx, z0 - 1) fixed, y = y0 - 1 .. y0 + TY
+      const int64_t hx = x0 + cbx - 1;
+      const int64_t hy0 = ring ? y0 + cby - 1 : y0 - 1, hz0 = ring ? z0 : z0 - 1;
+      const bool fixok = hx >= 0 && hx < sx && (ring ? (hy0 >= 0 && hy0 < sy) : (hz0 >= 0 && hz0 < sz));
+      const int64_t hstride = ring ? sx * sy : sx;
+      // the base may lie one row before the volume (y0 = 0); only addresses
+      // of voxels inside the volume are dereferenced
+      const T* hbase = in + (fixok ? hx + sx * (hy0 + sy * hz0) : 0);
+#pragma unroll
+      for (int k = 0; k < HV; k++) {
+        const bool along = ring ? (k < G::TZ && z0 + k < sz) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
+        hok[k] = fixok && along;
+        hv[k] = *(hok[k] ? hbase + k * hstride : in);
+      }
+    }
+  } else {
 #pragma unroll
     for (int k = 0; k < HV; k++) {
       int bx, by, bz;
-      if (HRUNS) {
-        bx = cbx;
-        by = ring ? cby : k;
-        bz = ring ? k + 1 : 0;
-      } else {
-        G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
-      }
+      G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
       const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-      const bool live = HRUNS ? (k < hlen) : (t + WG * k < G::HALO);
+      const bool live = t + WG * k < G::HALO;
       hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
       hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
     }
