@@ -169,8 +169,9 @@ __device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int
 
 // Lanes that share one tile in the per-tile global kernels: a tile holds a few
 // tens of requests or local roots, so a whole wave per tile would leave most
-// lanes idle.
-constexpr int TILE_LANES = 16;
+// lanes idle.  The label pass does less work per root and runs best wider.
+constexpr int TILE_LANES = 8;
+constexpr int LABEL_LANES = 16;
 
 // Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests.
 template <class G, int QK>
@@ -400,14 +401,14 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
 // its global root's key among the roots of its volume.
 template <class G>
 __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
-  const int sub = threadIdx.x % TILE_LANES;
-  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
-  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
+  const int sub = threadIdx.x % LABEL_LANES;
+  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / LABEL_LANES;
+  const int64_t ngroups = ((int64_t)gridDim.x * WG) / LABEL_LANES;
   for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
     const uint32_t R = p.TR[slot];
     if (R == 0) continue;
     const uint32_t base = p.nvol ? p.prefix[vol_geo(p, slot).key0 >> 5] : 0u;
-    for (uint32_t j = sub; j < R; j += TILE_LANES) {
+    for (uint32_t j = sub; j < R; j += LABEL_LANES) {
       const uint32_t g = p.RL[slot * G::TV + j];
       const uint32_t r = p.GP[g];
       const uint32_t wd = r >> 5;
@@ -459,11 +460,59 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
   }
 }
 
+// One workgroup finalizes FIN_TILES consecutive tiles: all their loads (root
+// indices and label tables) are issued before any store, so each wave keeps
+// more memory requests in flight.
+constexpr int FIN_TILES = 2;
+
 template <class OUT, class G>
 __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
-  __shared__ uint32_t lab[G::TV + 1];
-  const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
-  finalize_tile<OUT, G>(p, tile, tile, vol_geo(p, 0), out, lab);
+  static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
+  __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
+  const int t = threadIdx.x;
+  const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
+  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FIN_TILES;
+  const VolGeo v = vol_geo(p, 0);
+  uint32_t j[FIN_TILES][8];
+#pragma unroll
+  for (int n = 0; n < FIN_TILES; n++) {
+    const int64_t tile = first + n;
+    if (tile >= p.ntiles) break;
+    const uint32_t R = p.TR[tile];
+    if (R <= 255) {
+      const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + tile * G::TV)[t];
+#pragma unroll
+      for (int k = 0; k < 8; k++) j[n][k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
+    } else {
+      const uint4 w = reinterpret_cast<const uint4*>(p.J16 + tile * G::TV)[t];
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k = 0; k < 8; k++) j[n][k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    }
+    const uint32_t* __restrict__ LT = p.RL + tile * G::TV;
+    lab[n][0] = 0u;
+    for (uint32_t i = t; i < R; i += WG) lab[n][i + 1] = LT[i];
+  }
+  __syncthreads();
+  int lx, ly, lz;
+  G::voxel(t, 0, lx, ly, lz);
+#pragma unroll
+  for (int n = 0; n < FIN_TILES; n++) {
+    const int64_t tile = first + n;
+    if (tile >= p.ntiles) break;
+    const int64_t tz = tile / (v.ntx * v.nty);
+    const int64_t tr = tile - tz * v.ntx * v.nty;
+    const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
+    const int64_t gx = tx * G::TX + lx;
+    if (gx >= v.sx) continue;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      const int64_t gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
+      if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[n][j[n][k]];
+    }
+  }
 }
 
 // Batch: each tile writes its own volume's output in that volume's dtype.
@@ -994,6 +1043,7 @@ int run_global(cc3d_amd_job* j) {
   const GlobalParams& p = j->gp;
   hipStream_t s = j->stream;
   const unsigned wave_blocks = grid_for(j->g.ntiles, WG / TILE_LANES, 65536);
+  const unsigned label_blocks = grid_for(j->g.ntiles, WG / LABEL_LANES, 65536);
   {
   ProfScope ps_("k_requests", s);
   if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
@@ -1044,7 +1094,7 @@ int run_global(cc3d_amd_job* j) {
   hipLaunchKernelGGL(k_scan_c, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
   }
   ProfScope ps_l("k_labels", s);
-  hipLaunchKernelGGL((k_labels<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  hipLaunchKernelGGL((k_labels<G>), dim3(label_blocks), dim3(WG), 0, s, p);
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
 }
@@ -1295,7 +1345,7 @@ int cc3d_amd_label_finish(cc3d_amd_job* j, void* out, int out_dtype) {
   if (!j || !out) return fail(CC3D_AMD_ERR_INVALID, "NULL job or output");
   const Geometry& g = j->g;
   hipStream_t s = j->stream;
-  const dim3 grid((unsigned)g.ntiles), block(WG);
+  const dim3 grid((unsigned)((g.ntiles + FIN_TILES - 1) / FIN_TILES)), block(WG);
   int rc = CC3D_AMD_OK;
   ProfScope ps_("k_finalize", s);
 #define CC3D_F(OUT)                                                                                \
@@ -1668,6 +1718,7 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   }
   HIPCHECK(hipGetLastError());
   const unsigned wave_blocks = grid_for(b->ntiles, WG / TILE_LANES, 65536);
+  const unsigned label_blocks = grid_for(b->ntiles, WG / LABEL_LANES, 65536);
   {
     ProfScope ps_("k_requests", s);
     hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
@@ -1684,7 +1735,7 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   }
   {
     ProfScope ps_("k_labels", s);
-    hipLaunchKernelGGL((k_labels<Geo3>), dim3(wave_blocks), dim3(WG), 0, s, p);
+    hipLaunchKernelGGL((k_labels<Geo3>), dim3(label_blocks), dim3(WG), 0, s, p);
     hipLaunchKernelGGL(k_batch_N, dim3((b->n + 63) / 64), dim3(64), 0, s, p.vols, b->n, p.prefix, p.nwords, p.ctr,
                        p.ctr + 1);
   }
