@@ -351,13 +351,32 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint3
   return base + x - v;
 }
 
+// SCAN_WPT consecutive words from base (zeros past nwords), with 16-byte
+// loads when the whole group is in range.
+__device__ inline void load_words(const uint32_t* bits, int64_t base, int64_t nwords, uint32_t* w) {
+  if (base + SCAN_WPT <= nwords) {
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i += 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(bits + base + i);
+      w[i] = q.x;
+      w[i + 1] = q.y;
+      w[i + 2] = q.z;
+      w[i + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i++) w[i] = base + i < nwords ? bits[base + i] : 0u;
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_scan_a(const uint32_t* bits, int64_t nwords, uint32_t* bsum) {
   __shared__ uint32_t tmp[WG / 64];
   const int64_t base = (int64_t)blockIdx.x * SCAN_WPB + (int64_t)threadIdx.x * SCAN_WPT;
+  uint32_t w[SCAN_WPT];
+  load_words(bits, base, nwords, w);
   uint32_t c = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_WPT; i++)
-    if (base + i < nwords) c += __popc(bits[base + i]);
+  for (int i = 0; i < SCAN_WPT; i++) c += __popc(w[i]);
   uint32_t total;
   block_exclusive_scan(c, tmp, total);
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
@@ -382,18 +401,29 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
   __shared__ uint32_t tmp[WG / 64];
   const int64_t base = (int64_t)blockIdx.x * SCAN_WPB + (int64_t)threadIdx.x * SCAN_WPT;
   uint32_t w[SCAN_WPT];
+  load_words(bits, base, nwords, w);
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < SCAN_WPT; i++) {
-    w[i] = base + i < nwords ? __popc(bits[base + i]) : 0u;
+    w[i] = __popc(w[i]);
     c += w[i];
   }
   uint32_t total;
   uint32_t run = bsum[blockIdx.x] + block_exclusive_scan(c, tmp, total);
 #pragma unroll
   for (int i = 0; i < SCAN_WPT; i++) {
-    if (base + i < nwords) prefix[base + i] = run;
-    run += w[i];
+    const uint32_t v = w[i];
+    w[i] = run;
+    run += v;
+  }
+  if (base + SCAN_WPT <= nwords) {  // 16-byte stores for whole groups
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i += 4)
+      *reinterpret_cast<uint4*>(prefix + base + i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i++)
+      if (base + i < nwords) prefix[base + i] = w[i];
   }
 }
 
@@ -1200,6 +1230,10 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
       delete j;
       return fail(CC3D_AMD_ERR_INVALID, "workspace too small");
     }
+    if (reinterpret_cast<uintptr_t>(workspace) & 255) {
+      delete j;
+      return fail(CC3D_AMD_ERR_INVALID, "workspace must be 256-byte aligned");
+    }
     j->ws = static_cast<char*>(workspace);
   } else {
     void* p = nullptr;
@@ -1617,6 +1651,10 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     if (workspace_bytes < b->total) {
       delete b;
       return fail(CC3D_AMD_ERR_INVALID, "workspace too small");
+    }
+    if (reinterpret_cast<uintptr_t>(workspace) & 255) {
+      delete b;
+      return fail(CC3D_AMD_ERR_INVALID, "workspace must be 256-byte aligned");
     }
     b->ws = static_cast<char*>(workspace);
   } else {

@@ -757,6 +757,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     return L.rprefix[r >> 5] + (uint32_t)__popc(L.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
   };
   uint32_t* RLt = p.RL + slot * G::TV;
+  // Keys are below 2^32, so they can be formed in 32-bit arithmetic: the
+  // wrap-around of the intermediate terms cancels.  Box position (bx, by, bz)
+  // is voxel (x0 + bx - 1, y0 + by - 1, z0 + bz - 1).
+  const uint32_t kbox0 = (uint32_t)(x0 - 1) + (uint32_t)sx * ((uint32_t)(y0 - 1) + (uint32_t)sy * (uint32_t)(z0 - 1));
   for (int w = t; w < RW; w += WG) {
     uint32_t bits = L.rbits[w];
     uint32_t rk = L.rprefix[w];
@@ -765,7 +769,13 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       bits &= bits - 1u;
       int bx, by, bz;
       G::template nid_box<QK>(32 * w + bpos, bx, by, bz);
-      const uint32_t key = gkey<QK>(x0 + bx - 1, y0 + by - 1, IS3D ? z0 + bz - 1 : 0, sx, sy, p.osx) + keybase;
+      uint32_t key;
+      if (QK == Q_BLOCK8)
+        key = gkey<QK>(x0 + bx - 1, y0 + by - 1, 0, sx, sy, p.osx) + keybase;
+      else if (IS3D)
+        key = kbox0 + (uint32_t)bx + (uint32_t)sx * ((uint32_t)by + (uint32_t)sy * (uint32_t)bz) + keybase;
+      else
+        key = kbox0 + (uint32_t)sx * (uint32_t)sy + (uint32_t)bx + (uint32_t)sx * (uint32_t)by + keybase;
       RLt[rk++] = key;
       p.GP[key] = key;
     }

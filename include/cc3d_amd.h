@@ -78,7 +78,8 @@ typedef struct {
 size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity,
                                 int binary_image, int flags);
 
-/* Stage 1.  workspace may be NULL (the library allocates it stream-ordered).
+/* Stage 1.  workspace may be NULL (the library allocates it stream-ordered);
+ * a caller workspace must be 256-byte aligned.
  * With periodic_boundary on a multilabel volume whose foreground is a single
  * row, the wrap-around unions are skipped, as the binding's single-row
  * shortcut (cc3d/fastcc3d.pyx:469-470, :644-679) ignores periodicity, unless
@@ -122,7 +123,8 @@ typedef struct cc3d_amd_batch cc3d_amd_batch;
 /* Bytes of scratch a batch needs (0 if a volume is unsupported). */
 size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int binary_image);
 /* Plans a batch (no launches).  workspace may be NULL (allocated here and
- * freed by cc3d_amd_batch_free).  All later launches go to `stream`. */
+ * freed by cc3d_amd_batch_free); a caller workspace must be 256-byte aligned.
+ * All later launches go to `stream`. */
 int cc3d_amd_batch_create(cc3d_amd_batch** batch, const cc3d_amd_volume* vols, int n, int in_dtype,
                           int binary_image, void* workspace, size_t workspace_bytes, void* stream);
 /* Output buffer and dtype (1/2/3) of every volume; uploaded stream-ordered. */
