@@ -50,6 +50,35 @@ __global__ void k_init(Counters* c, uint32_t* bits, int64_t nwords) {
   for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
 }
 
+// Union tables (UTab in common.hpp), one copy per device, built once by
+// k_build_utab: each thread grows one (border class, link mask) entry.
+static __device__ uint16_t g_utab[UTAB_TOTAL];
+
+template <int NB>
+__global__ void k_build_utab(uint16_t* tab) {
+  using U = UTab<NB>;
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= (U::ROWS << U::N)) return;
+  const int cls = e >> U::N;
+  const uint32_t lk = (uint32_t)e & ((1u << U::N) - 1u);
+  const Border bd = class_border<Nbhd<NB>::IS3D>(cls);
+  uint32_t conn = 0, u = 0;
+  while (lk & ~conn) {
+    const int i = __builtin_ctz(lk & ~conn);
+    u |= 1u << i;
+    uint32_t front = 1u << i;
+    conn |= front;
+    while (front) {
+      uint32_t nb = 0;
+      for (int j = 0; j < U::N; j++)
+        if ((front >> j) & 1u) nb |= cover_from<NB>(j, bd);
+      front = nb & lk & ~conn;
+      conn |= front;
+    }
+  }
+  tab[U::OFF + e] = (uint16_t)u;
+}
+
 // Batch: reset nc counters and clear the root bitmask (a fill kernel, so the
 // sequence can be captured in a graph).
 __global__ void k_batch_init(Counters* c, int nc, uint32_t* bits, int64_t nwords) {
@@ -867,6 +896,37 @@ static int fail(int code, const std::string& msg) {
     if (_e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
   } while (0)
 
+// Device address of the current device's union tables; the first call on a
+// device builds them on a private stream and waits for it.
+static std::mutex g_utab_mu;
+static std::map<int, const uint16_t*> g_utab_dev;
+
+static int utab_ready(const uint16_t** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_utab_mu);
+  auto it = g_utab_dev.find(dev);
+  if (it != g_utab_dev.end()) {
+    *out = it->second;
+    return CC3D_AMD_OK;
+  }
+  void* ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_utab)));
+  uint16_t* tab = static_cast<uint16_t*>(ptr);
+  hipStream_t s = nullptr;
+  HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipLaunchKernelGGL(k_build_utab<26>, dim3((UTab<26>::ROWS << UTab<26>::N) / 256), dim3(256), 0, s, tab);
+  hipLaunchKernelGGL(k_build_utab<18>, dim3((UTab<18>::ROWS << UTab<18>::N) / 256), dim3(256), 0, s, tab);
+  hipLaunchKernelGGL(k_build_utab<8>, dim3(1), dim3(256), 0, s, tab);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("building the union tables: ") + hipGetErrorString(e));
+  g_utab_dev[dev] = tab;
+  *out = tab;
+  return CC3D_AMD_OK;
+}
+
 // Optional per-kernel timing with HIP events on the launch stream (enabled by
 // cc3d_amd_profile_enable; read back by cc3d_amd_profile_collect).
 struct ProfRec {
@@ -1308,6 +1368,11 @@ int cc3d_amd_label_begin(cc3d_amd_job** out_job, const void* in, int in_dtype, i
 #undef CC3D_DT_SWITCH
 
   LocalParams lp;
+  rc = utab_ready(&lp.utab);
+  if (rc != CC3D_AMD_OK) {
+    cc3d_amd_job_free(j);
+    return rc;
+  }
   lp.in = in;
   lp.sx = g.sx; lp.sy = g.sy; lp.sz = g.sz; lp.osx = g.osx;
   lp.ntx = g.ntx; lp.nty = g.nty; lp.ntz = g.ntz; lp.ntiles = g.ntiles;
@@ -1700,6 +1765,11 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
   gp.vtile0 = reinterpret_cast<const int64_t*>(ws + b->oT0);
   gp.nvol = n;
   LocalParams& lp = b->lp;
+  rc = utab_ready(&lp.utab);
+  if (rc != CC3D_AMD_OK) {
+    cc3d_amd_batch_free(b);
+    return rc;
+  }
   lp.ntiles = b->ntiles;
   lp.delta_bits = 0;
   lp.minmax = nullptr;

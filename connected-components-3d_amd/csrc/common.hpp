@@ -501,4 +501,33 @@ __device__ inline uint32_t cover_from(int j, const Border& b) {
   return Nbhd<NB>::adj(j) & proc_mask<Nbhd<NB>::IS3D>(j, b);
 }
 
+// Border class of a tile voxel: x (inside / first / last column of the tile)
+// + 3 * y (inside / first / last row) + 9 * z (inside / first plane).  2D
+// tiles use classes 0..5 (their last row borders nothing backward).
+__host__ __device__ constexpr int border_class(bool xlo, bool xhi, bool ylo, bool yhi, bool zlo) {
+  return (xlo ? 1 : xhi ? 2 : 0) + 3 * (ylo ? 1 : yhi ? 2 : 0) + 9 * (zlo ? 1 : 0);
+}
+
+template <bool IS3D>
+__device__ inline Border class_border(int cls) {
+  const int xc = cls % 3, yc = (cls / 3) % 3;
+  return make_border<IS3D>(xc == 1, xc == 2, yc == 1, yc == 2, cls >= 9);
+}
+
+// Union tables of the transitive connectivities (26, 18 and 2D 8).  Entry
+// (class, lk) lists the backward positions that a voxel of that border class
+// whose linked positions are lk unions with: the first linked position, then
+// growth over cover_from marks every linked position adjacent (through a pair
+// unioned in the tile's structure) to a connected one, and the next linked
+// position that growth cannot reach is the next union.  Class 0 (no border)
+// also serves halo voxels, whose lk holds only eligible positions.  Rows of
+// 2^N entries; the three tables share one device array of UTAB_TOTAL entries.
+template <int NB>
+struct UTab {
+  static constexpr int N = Nbhd<NB>::N;
+  static constexpr int ROWS = Nbhd<NB>::IS3D ? 18 : 6;
+  static constexpr int OFF = NB == 26 ? 0 : NB == 18 ? (18 << 13) : (18 << 13) + (18 << 9);
+};
+constexpr int UTAB_TOTAL = (18 << 13) + (18 << 9) + (6 << 4);
+
 }  // namespace cc3d_amd

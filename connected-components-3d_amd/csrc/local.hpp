@@ -72,6 +72,7 @@ struct LocalParams {
   uint32_t* REQ;
   uint32_t* TQ;
   TileCnt* tcnt;
+  const uint16_t* utab;  // union tables (UTab), UTAB_TOTAL entries
 };
 
 // Offsets of backward position i in the staged box and in the tile's node
@@ -133,8 +134,8 @@ __device__ inline bool column_link(T cur, T prev, T delta, T gmin, T gmax) {
 // no linked backward neighbour (the reference gives p a new provisional label).
 template <class T, int MODE, int NB, int QK, class G, bool HALO>
 __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn, int bx, int by, int bz,
-                                 T cur, const Border& bd, uint32_t elig, bool col, T delta, T gmin, T gmax,
-                                 bool row0) {
+                                 T cur, const Border& bd, const uint16_t* ut, uint32_t elig, bool col, T delta,
+                                 T gmin, T gmax, bool row0) {
   using NH = Nbhd<NB>;
   constexpr bool IS3D = G::IS3D;
   auto val = [&](int i) -> T { return sv[b + box_off<G>(i)]; };
@@ -229,14 +230,13 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
       return (int)lK + (int)lM;
     }
   } else if constexpr (MODE != M_CONT) {
-    // 26 / 18 / 8: transitive links.  Read every neighbour into a link mask,
-    // then grow the set of neighbours known to be connected to p: a linked
-    // neighbour adjacent (through a pair unioned in this tile's structure) to
-    // a connected one is connected too.  Only a linked neighbour the growth
-    // cannot reach is unioned, and the growth restarts from it.
+    // 26 / 18 / 8: transitive links.  Read every neighbour into a link mask;
+    // the union table row of p's border class (ut) lists the positions to
+    // union with (UTab in common.hpp).  A run member's link to neighbour 0 is
+    // in place already.
     constexpr uint32_t ALLN = (1u << NH::N) - 1u;
     const uint32_t okm = HALO ? (elig & ALLN) : ALLN;
-    uint32_t lk = 0;
+    uint32_t lk = col ? 1u : 0u;
 #pragma unroll
     for (int i = 0; i < NH::N; i++) {
       if (col && i == 0) continue;
@@ -245,26 +245,11 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
       const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
       lk |= l ? (1u << i) : 0u;
     }
-    uint32_t conn = col ? 1u : 0u;
-    lk |= conn;
-    uint32_t front = conn;
-    uint32_t cv[NH::N];  // cover masks, computed once for the growth rounds
-#pragma unroll
-    for (int j = 0; j < NH::N; j++) cv[j] = cover(j);
-    while (true) {
-      while (front) {
-        uint32_t nb = 0;
-#pragma unroll
-        for (int j = 0; j < NH::N; j++) nb |= ((front >> j) & 1u) ? cv[j] : 0u;
-        front = nb & lk & ~conn;
-        conn |= front;
-      }
-      const uint32_t rest = lk & ~conn;
-      if (!rest) break;
-      const int i = __builtin_ctz(rest);
+    uint32_t u = lk ? (uint32_t)ut[lk] & ~(col ? 1u : 0u) : 0u;
+    while (u) {
+      const int i = __builtin_ctz(u);
+      u &= u - 1u;
       a = lunion(par, a, nd_dyn(i));
-      conn |= 1u << i;
-      front = 1u << i;
     }
     return lk ? 1 : 0;
   } else {
@@ -382,6 +367,9 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 
   T* sv = L.sv;
   uint32_t* par = L.par;
+  // union table of this connectivity, class 0 (transitive connectivities only)
+  constexpr bool UT = MODE != M_CONT && (NB == 26 || NB == 18 || NB == 8) && QK != Q_CONT8;
+  const uint16_t* ut0 = UT ? p.utab + UTab<UT ? NB : 26>::OFF : nullptr;
   uint16_t* queue = L.queue;
   uint16_t* hqueue = L.hqueue;
   uint32_t& qcount = L.qcount;
@@ -600,7 +588,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
       const int b = G::box(bx, by, bz);
-      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{},
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{}, ut0,
                                            halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
     }
   } else if constexpr (PIECES) {
@@ -611,7 +599,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       const T cur = sv[b];
       if (!fg(cur)) continue;
       link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)G::template nid<QK>(bx, by, bz), bx, by, bz, cur,
-                                           Border{}, halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
+                                           Border{}, ut0, halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
     }
   }
 
@@ -706,9 +694,12 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     const int bx = vx + 1, by = vy + 1, bz = IS3D ? vz + 1 : 0;
     const int b = G::box(bx, by, bz);
     const T cur = sv[b];
-    const Border bd = make_border<IS3D>(vx == 0, vx == G::TX - 1, vy == 0, IS3D && vy == G::TY - 1, IS3D && vz == 0);
+    const bool xlo = vx == 0, xhi = vx == G::TX - 1, ylo = vy == 0, yhi = IS3D && vy == G::TY - 1;
+    const bool zlo = IS3D && vz == 0;
+    const Border bd = make_border<IS3D>(xlo, xhi, ylo, yhi, zlo);
+    const uint16_t* ut = ut0 + (border_class(xlo, xhi, ylo, yhi, zlo) << Nbhd<NB>::N);
     const uint32_t pn = (uint32_t)G::template nid<QK>(bx, by, bz);
-    const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, 0u, colq, delta,
+    const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, ut, 0u, colq, delta,
                                                           gmin, gmax, y0 + vy == 0);
     if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
   }
