@@ -88,7 +88,7 @@ struct TileGeo {
   static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0 && (TZ & (TZ - 1)) == 0,
                 "tile sides are powers of two");
 
-  __device__ static inline int box(int bx, int by, int bz) { return bx + BX * (by + BY * bz); }
+  __host__ __device__ static constexpr int box(int bx, int by, int bz) { return bx + BX * (by + BY * bz); }
   __device__ static inline bool is_tile(int bx, int by, int bz) {
     return bx >= 1 && bx <= TX && by >= 1 && by <= TY && (!IS3D || bz >= 1);
   }
@@ -151,7 +151,7 @@ struct TileGeo {
   static constexpr int NIDS = QK == Q_BLOCK8 ? 4 * BW * ((BY >> 1) + 1) : BOXN;
   static constexpr int NIDS_MAX = NIDS<Q_BLOCK8> > BOXN ? NIDS<Q_BLOCK8> : BOXN;
   template <int QK>
-  __device__ static inline int nid(int bx, int by, int bz) {
+  __host__ __device__ static constexpr int nid(int bx, int by, int bz) {
     if (QK == Q_BLOCK8) {
       const int xb = bx + 1, yb = by + 1;
       return (((yb >> 1) * BW + (xb >> 1)) << 2) | ((yb & 1) << 1) | (xb & 1);

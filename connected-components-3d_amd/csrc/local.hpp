@@ -127,6 +127,54 @@ __device__ inline bool column_link(T cur, T prev, T delta, T gmin, T gmax) {
   }
 }
 
+// Node of backward position i (a run-time index) of the voxel at box
+// (bx, by, bz) with node pn; offsets are unpacked from 2-bit fields.
+template <int QK, class G>
+__device__ inline uint32_t node_at(uint32_t pn, int bx, int by, int bz, int i) {
+  constexpr bool IS3D = G::IS3D;
+  constexpr uint32_t PX = packed_offsets<IS3D>(0), PY = packed_offsets<IS3D>(1), PZ = packed_offsets<IS3D>(2);
+  const int dx = (int)((PX >> (2 * i)) & 3u) - 1;
+  const int dy = (int)((PY >> (2 * i)) & 3u) - 1;
+  const int dz = (int)((PZ >> (2 * i)) & 3u) - 1;
+  if (QK != Q_BLOCK8) return pn + dx + G::BX * (dy + G::BY * dz);
+  return G::template nid<QK>(bx + dx, by + dy, bz + dz);
+}
+
+// Link mask of a foreground voxel (box index b, value cur) under a
+// transitive connectivity (26/18/8): bit i when backward position i is in
+// okm and linked.  A run member (col) has bit 0 set without reading it.
+template <class T, int MODE, int NB, class G>
+__device__ inline uint32_t link_mask(const T* sv, int b, T cur, uint32_t okm, bool col) {
+  uint32_t lk = col ? 1u : 0u;
+#pragma unroll
+  for (int i = 0; i < Nbhd<NB>::N; i++) {
+    if (col && i == 0) continue;
+    if (!((okm >> i) & 1u)) continue;
+    const T q = sv[b + box_off<G>(i)];
+    const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
+    lk |= l ? (1u << i) : 0u;
+  }
+  return lk;
+}
+
+// Raw buffer loads (gfx9 buffer resource, 32-bit byte offsets): an offset
+// at or above the resource's record count reads 0 without touching memory,
+// which is background for every voxel type.
+constexpr uint32_t OOB = 0x80000000u;
+constexpr int BUF_DW3 = 0x00020000;
+template <class T>
+__device__ inline T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 1) {
+    return __builtin_bit_cast(T, (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0));
+  } else if constexpr (sizeof(T) == 2) {
+    return __builtin_bit_cast(T, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+}
+
 // Unions for one foreground voxel p (box index b, node pn, box coordinates
 // bx, by, bz).  HALO: a halo voxel, restricted to the positions in `elig`
 // (halo voxels of the same neighbouring tile).  COL: p continues a run, so
@@ -146,14 +194,7 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
   };
   auto cover = [&](int i) -> uint32_t { return HALO ? NH::adj(i) : cover_from<NB>(i, bd); };
   // node of position i for a run-time i (offsets unpacked from 2-bit fields)
-  auto nd_dyn = [&](int i) -> uint32_t {
-    constexpr uint32_t PX = packed_offsets<IS3D>(0), PY = packed_offsets<IS3D>(1), PZ = packed_offsets<IS3D>(2);
-    const int dx = (int)((PX >> (2 * i)) & 3u) - 1;
-    const int dy = (int)((PY >> (2 * i)) & 3u) - 1;
-    const int dz = (int)((PZ >> (2 * i)) & 3u) - 1;
-    if (QK != Q_BLOCK8) return pn + dx + G::BX * (dy + G::BY * dz);
-    return G::template nid<QK>(bx + dx, by + dy, bz + dz);
-  };
+  auto nd_dyn = [&](int i) -> uint32_t { return node_at<QK, G>(pn, bx, by, bz, i); };
   auto ok = [&](int i) -> bool { return !HALO || ((elig >> i) & 1u); };
   uint32_t a = pn;
 
@@ -235,17 +276,8 @@ __device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn,
     // union with (UTab in common.hpp).  A run member's link to neighbour 0 is
     // in place already.
     constexpr uint32_t ALLN = (1u << NH::N) - 1u;
-    const uint32_t okm = HALO ? (elig & ALLN) : ALLN;
-    uint32_t lk = col ? 1u : 0u;
-#pragma unroll
-    for (int i = 0; i < NH::N; i++) {
-      if (col && i == 0) continue;
-      if (!((okm >> i) & 1u)) continue;
-      const T q = val(i);
-      const bool l = MODE == M_MULTI ? (q == cur) : fg(q);
-      lk |= l ? (1u << i) : 0u;
-    }
-    uint32_t u = lk ? (uint32_t)ut[lk] & ~(col ? 1u : 0u) : 0u;
+    const uint32_t lk = link_mask<T, MODE, NB, G>(sv, b, cur, HALO ? (elig & ALLN) : ALLN, col);
+    uint32_t u = (uint32_t)ut[lk] & ~(col ? 1u : 0u);
     while (u) {
       const int i = __builtin_ctz(u);
       u &= u - 1u;
@@ -413,30 +445,14 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 
   // 1. stage the column (registers + LDS) and the halo; voxels outside the
   //    volume read as background.  Column runs get their parent here.  All
-  //    global loads (column, its predecessor, halo) are issued before any
-  //    value is used: out-of-volume voxels load a valid address and are
-  //    replaced by background afterwards, so one wait covers every load.
+  //    loads (column, its predecessor, halo) are issued before any value is
+  //    used, so one wait covers them.
   constexpr int HLEN = G::TZ > G::BY ? G::TZ : G::BY;
   constexpr int HV = HRUNS ? HLEN : HPT;
   T col[PT];
   T hv[HV];
+  T pv;
   uint32_t runs = 0;  // bit k: voxel k continues the run of its predecessor
-  bool inb[PT];
-  // one base address per column, stepped by a stride that is the same for
-  // every thread; the bounds along the column are per tile
-  const int64_t cstride = IS3D ? sx * sy : sx;                   // voxel k -> k + 1
-  const int64_t ccount = IS3D ? sz - z0 : sy - (y0 + ly0);       // voxels left in the volume
-  const bool colok = gx < sx && y0 + ly0 < sy;  // the column's first voxel is in the volume
-  const T* cbase = in + (colok ? gx + sx * ((y0 + ly0) + sy * z0) : 0);
-#pragma unroll
-  for (int k = 0; k < PT; k++) {
-    inb[k] = colok && k < ccount;
-    col[k] = *(inb[k] ? cbase + k * cstride : in);
-  }
-  // predecessor of voxel 0: (x, y, z0 - 1) in 3D, (x, y - 1) in 2D
-  const int64_t py = IS3D ? y0 + ly0 : y0 + ly0 - 1, pz = IS3D ? z0 - 1 : 0;
-  const bool pok = gx < sx && py >= 0 && py < sy && pz >= 0 && pz < sz;
-  T pv = in[pok ? gx + sx * (py + sy * pz) : 0];
   // 3D halo as columns (HRUNS): ring position r (threads 0..RING-1) along z,
   // bottom plane column bx (threads RING..RING+BX-1) along y.  Otherwise
   // thread t stages halo voxels t, t + WG, ...
@@ -462,46 +478,109 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
   }
   const int hlen = ring ? G::TZ : G::BY;
-  bool hok[HV];
 #pragma unroll
-  for (int k = 0; k < HV; k++) {
-    hok[k] = false;
-    hv[k] = T(0);
-  }
-  if constexpr (HRUNS) {
-    if (hcol) {  // waves holding no halo column skip the address work
-      // ring column: (hx, hy) fixed, z = z0 .. z0 + TZ - 1; bottom column:
-      // (hx, z0 - 1) fixed, y = y0 - 1 .. y0 + TY
-      const int64_t hx = x0 + cbx - 1;
-      const int64_t hy0 = ring ? y0 + cby - 1 : y0 - 1, hz0 = ring ? z0 : z0 - 1;
-      const bool fixok = hx >= 0 && hx < sx && (ring ? (hy0 >= 0 && hy0 < sy) : (hz0 >= 0 && hz0 < sz));
-      const int64_t hstride = ring ? sx * sy : sx;
-      // the base may lie one row before the volume (y0 = 0); only addresses
-      // of voxels inside the volume are dereferenced
-      const T* hbase = in + (fixok ? hx + sx * (hy0 + sy * hz0) : 0);
+  for (int k = 0; k < HV; k++) hv[k] = T(0);
+  // The box (tile plus halo) spans fewer than 2^31 bytes of the volume for
+  // any realistic plane size: then every voxel is one raw buffer load at a
+  // 32-bit byte offset from box position 0, and voxels outside the volume
+  // read 0 (offset OOB, or past the volume's end, which the record count
+  // covers).  Otherwise 64-bit addresses with explicit bounds.
+  const bool buf_ok =
+    (uint64_t)(G::BX + sx * G::BY + (IS3D ? sx * sy * G::BZ : 0)) * sizeof(T) < (uint64_t)OOB;
+  if (buf_ok) {
+    const int64_t base_el = (x0 - 1) + sx * ((y0 - 1) + (IS3D ? sy * (z0 - 1) : 0));
+    const uint64_t rem = (uint64_t)((IS3D ? sx * sy * sz : sx * sy) - base_el) * sizeof(T);
+    const int nrec = (int)(rem < OOB ? rem : OOB - 1u);
+    const char* bp = reinterpret_cast<const char*>(in) + base_el * (int64_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(bp), 0, nrec, BUF_DW3);
+    constexpr uint32_t SZ = sizeof(T);
+    const uint32_t RB = (uint32_t)sx * SZ, PB = IS3D ? (uint32_t)(sx * sy) * SZ : 0u;
+    const uint32_t CST = IS3D ? PB : RB;                                          // column step
+    const uint32_t c0 = (uint32_t)(lx + 1) * SZ + (uint32_t)(ly0 + 1) * RB + PB;  // column voxel 0
+    const uint32_t coff = (gx < sx && y0 + ly0 < sy) ? c0 : OOB;
+#pragma unroll
+    for (int k = 0; k < PT; k++) col[k] = bload<T>(rs, coff + (uint32_t)k * CST);
+    // predecessor of voxel 0: (x, y, z0 - 1) in 3D, (x, y - 1) in 2D
+    const bool pok = gx < sx && (IS3D ? (y0 + ly0 < sy && z0 > 0) : (y0 + ly0 > 0));
+    pv = bload<T>(rs, pok ? c0 - CST : OOB);
+    if constexpr (HRUNS) {
+      if (hcol) {
+        // ring column: box (cbx, cby, 1 + k); bottom column: box (cbx, k, 0)
+        const int64_t hx = x0 + cbx - 1, hy = y0 + cby - 1;
+        const bool fixok = hx >= 0 && hx < sx && (ring ? (hy >= 0 && hy < sy) : z0 > 0);
+        const uint32_t h0 = (uint32_t)cbx * SZ + (ring ? (uint32_t)cby * RB + PB : 0u);
+        const uint32_t hst = ring ? PB : RB;
+#pragma unroll
+        for (int k = 0; k < HV; k++) {
+          const bool along = ring ? (k < G::TZ) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
+          hv[k] = bload<T>(rs, (fixok && along) ? h0 + (uint32_t)k * hst : OOB);
+        }
+      }
+    } else {
 #pragma unroll
       for (int k = 0; k < HV; k++) {
-        const bool along = ring ? (k < G::TZ && z0 + k < sz) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
-        hok[k] = fixok && along;
-        hv[k] = *(hok[k] ? hbase + k * hstride : in);
+        const bool live = t + WG * k < G::HALO;
+        int bx, by, bz;
+        G::halo_coords(live ? t + WG * k : 0, bx, by, bz);
+        const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+        const bool ok = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0;
+        hv[k] = bload<T>(rs, ok ? (uint32_t)bx * SZ + (uint32_t)by * RB + (uint32_t)bz * PB : OOB);
       }
     }
   } else {
+    // All global loads are issued before any value is used: out-of-volume
+    // voxels load a valid address and are replaced by background afterwards.
+    bool inb[PT];
+    // one base address per column, stepped by a stride that is the same for
+    // every thread; the bounds along the column are per tile
+    const int64_t cstride = IS3D ? sx * sy : sx;                   // voxel k -> k + 1
+    const int64_t ccount = IS3D ? sz - z0 : sy - (y0 + ly0);       // voxels left in the volume
+    const bool colok = gx < sx && y0 + ly0 < sy;  // the column's first voxel is in the volume
+    const T* cbase = in + (colok ? gx + sx * ((y0 + ly0) + sy * z0) : 0);
 #pragma unroll
-    for (int k = 0; k < HV; k++) {
-      int bx, by, bz;
-      G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
-      const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-      const bool live = t + WG * k < G::HALO;
-      hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
-      hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
+    for (int k = 0; k < PT; k++) {
+      inb[k] = colok && k < ccount;
+      col[k] = *(inb[k] ? cbase + k * cstride : in);
     }
+    const int64_t py = IS3D ? y0 + ly0 : y0 + ly0 - 1, pz = IS3D ? z0 - 1 : 0;
+    const bool pok = gx < sx && py >= 0 && py < sy && pz >= 0 && pz < sz;
+    pv = in[pok ? gx + sx * (py + sy * pz) : 0];
+    bool hok[HV];
+#pragma unroll
+    for (int k = 0; k < HV; k++) hok[k] = false;
+    if constexpr (HRUNS) {
+      if (hcol) {  // waves holding no halo column skip the address work
+        const int64_t hx = x0 + cbx - 1;
+        const int64_t hy0 = ring ? y0 + cby - 1 : y0 - 1, hz0 = ring ? z0 : z0 - 1;
+        const bool fixok = hx >= 0 && hx < sx && (ring ? (hy0 >= 0 && hy0 < sy) : (hz0 >= 0 && hz0 < sz));
+        const int64_t hstride = ring ? sx * sy : sx;
+        // the base may lie one row before the volume (y0 = 0); only addresses
+        // of voxels inside the volume are dereferenced
+        const T* hbase = in + (fixok ? hx + sx * (hy0 + sy * hz0) : 0);
+#pragma unroll
+        for (int k = 0; k < HV; k++) {
+          const bool along = ring ? (k < G::TZ && z0 + k < sz) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
+          hok[k] = fixok && along;
+          hv[k] = *(hok[k] ? hbase + k * hstride : in);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < HV; k++) {
+        int bx, by, bz;
+        G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
+        const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
+        const bool live = t + WG * k < G::HALO;
+        hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
+        hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PT; k++) col[k] = inb[k] ? col[k] : T(0);
+    pv = pok ? pv : T(0);
+#pragma unroll
+    for (int k = 0; k < HV; k++) hv[k] = hok[k] ? hv[k] : T(0);
   }
-#pragma unroll
-  for (int k = 0; k < PT; k++) col[k] = inb[k] ? col[k] : T(0);
-  pv = pok ? pv : T(0);
-#pragma unroll
-  for (int k = 0; k < HV; k++) hv[k] = hok[k] ? hv[k] : T(0);
   {
     uint32_t start = QK != Q_BLOCK8 ? (uint32_t)(b0 - CSTEP)
                                     : (uint32_t)G::template nid<QK>(lx + 1, ly0, 0);  // predecessor node
@@ -579,17 +658,21 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
   __syncthreads();
 
-  // 2. halo pieces: links between halo voxels of the same neighbouring tile
+  // 2. halo pieces: links between halo voxels of the same neighbouring tile.
+  //    Under HRUNS the link mask and union-table load of the thread's first
+  //    halo run start are issued here and its unions made after step 3, which
+  //    does not depend on them, so the load's latency overlaps step 3.
+  uint32_t h_nh = 0, h_b = 0, h_u = 0;
+  int h_bx = 0, h_by = 0, h_bz = 0;
   if constexpr (HRUNS) {
-    const uint32_t nh = hcount;
-#pragma unroll 1
-    for (uint32_t q = t; q < nh; q += WG) {
-      const int h = hqueue[q];
-      int bx, by, bz;
-      G::halo_coords(h, bx, by, bz);
-      const int b = G::box(bx, by, bz);
-      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{}, ut0,
-                                           halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
+    h_nh = hcount;
+    if ((uint32_t)t < h_nh) {
+      G::halo_coords(hqueue[t], h_bx, h_by, h_bz);
+      h_b = (uint32_t)G::box(h_bx, h_by, h_bz);
+      constexpr uint32_t ALLN = (1u << Nbhd<NB>::N) - 1u;
+      const uint32_t lk =
+        link_mask<T, MODE, NB, G>(sv, (int)h_b, sv[h_b], halo_elig<G>(h_bx, h_by, h_bz) & ALLN, false);
+      h_u = ut0[lk];  // class 0: halo pairs of one neighbouring tile are all unioned here
     }
   } else if constexpr (PIECES) {
     for (int h = t; h < G::HALO; h += WG) {
@@ -659,6 +742,25 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       if ((needB >> k) & 1u) queue[G::TV - 1 - ib++] = ps;
     }
   }
+  if constexpr (HRUNS) {
+    // unions of the halo run start prepared in step 2, then any further ones
+    if ((uint32_t)t < h_nh) {
+      uint32_t a = h_b, u = h_u;
+      while (u) {
+        const int i = __builtin_ctz(u);
+        u &= u - 1u;
+        a = lunion(par, a, node_at<QK, G>(h_b, h_bx, h_by, h_bz, i));
+      }
+    }
+#pragma unroll 1
+    for (uint32_t q = t + WG; q < h_nh; q += WG) {
+      int bx, by, bz;
+      G::halo_coords(hqueue[q], bx, by, bz);
+      const int b = G::box(bx, by, bz);
+      link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{}, ut0,
+                                           halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
+    }
+  }
   __syncthreads();
 
   // halo pieces are the roots left by step 2 (without pieces every
@@ -686,37 +788,122 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   __syncthreads();
 
   // 4. queued voxels
+  if constexpr (UT) {
+    // Transitive connectivities: link mask, then the union table.
+    struct Item {
+      uint32_t pn, lk, u, cm;
+      int bx, by, bz;
+    };
+    auto prep = [&](uint32_t q) -> Item {
+      const bool colq = q >= nA;
+      const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
+      const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
+      Item it;
+      it.bx = vx + 1;
+      it.by = vy + 1;
+      it.bz = IS3D ? vz + 1 : 0;
+      const int b = G::box(it.bx, it.by, it.bz);
+      it.pn = (uint32_t)G::template nid<QK>(it.bx, it.by, it.bz);
+      it.lk = link_mask<T, MODE, NB, G>(sv, b, sv[b], (1u << Nbhd<NB>::N) - 1u, colq);
+      const int cls = border_class(vx == 0, vx == G::TX - 1, vy == 0, IS3D && vy == G::TY - 1, IS3D && vz == 0);
+      it.u = ut0[(cls << Nbhd<NB>::N) | it.lk];
+      it.cm = colq ? ~1u : ~0u;  // a run member's link to neighbour 0 is in place
+      return it;
+    };
+    auto unions = [&](const Item& it) {
+      uint32_t a = it.pn, u = it.u & it.cm;
+      while (u) {
+        const int i = __builtin_ctz(u);
+        u &= u - 1u;
+        a = lunion(par, a, node_at<QK, G>(it.pn, it.bx, it.by, it.bz, i));
+      }
+      if (QK != Q_BLOCK8 && it.lk == 0) my_prov++;  // block labels are counted separately
+    };
+    // Two entries per round: the second one's table load is issued first and
+    // the first one's right after, so both latencies overlap.
 #pragma unroll 1
-  for (uint32_t q = t; q < nQ; q += WG) {
-    const bool colq = q >= nA;
-    const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
-    const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
-    const int bx = vx + 1, by = vy + 1, bz = IS3D ? vz + 1 : 0;
-    const int b = G::box(bx, by, bz);
-    const T cur = sv[b];
-    const bool xlo = vx == 0, xhi = vx == G::TX - 1, ylo = vy == 0, yhi = IS3D && vy == G::TY - 1;
-    const bool zlo = IS3D && vz == 0;
-    const Border bd = make_border<IS3D>(xlo, xhi, ylo, yhi, zlo);
-    const uint16_t* ut = ut0 + (border_class(xlo, xhi, ylo, yhi, zlo) << Nbhd<NB>::N);
-    const uint32_t pn = (uint32_t)G::template nid<QK>(bx, by, bz);
-    const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, ut, 0u, colq, delta,
-                                                          gmin, gmax, y0 + vy == 0);
-    if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
+    for (uint32_t q = t; q < nQ; q += 2 * WG) {
+      const uint32_t q2 = q + WG;
+      Item i2;  // left unset past the end
+      if (q2 < nQ) i2 = prep(q2);
+      const Item i1 = prep(q);
+      unions(i1);
+      if (q2 < nQ) unions(i2);
+    }
+  } else {
+#pragma unroll 1
+    for (uint32_t q = t; q < nQ; q += WG) {
+      const bool colq = q >= nA;
+      const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
+      const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
+      const int bx = vx + 1, by = vy + 1, bz = IS3D ? vz + 1 : 0;
+      const int b = G::box(bx, by, bz);
+      const T cur = sv[b];
+      const bool xlo = vx == 0, xhi = vx == G::TX - 1, ylo = vy == 0, yhi = IS3D && vy == G::TY - 1;
+      const bool zlo = IS3D && vz == 0;
+      const Border bd = make_border<IS3D>(xlo, xhi, ylo, yhi, zlo);
+      const uint16_t* ut = ut0 + (border_class(xlo, xhi, ylo, yhi, zlo) << Nbhd<NB>::N);
+      const uint32_t pn = (uint32_t)G::template nid<QK>(bx, by, bz);
+      const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, ut, 0u, colq, delta,
+                                                            gmin, gmax, y0 + vy == 0);
+      if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
+    }
   }
   __syncthreads();
 
-  // 5. resolve roots (run members after the first share their run start's
-  //    root) and mark the roots that own tile voxels
+  // 5. resolve roots and mark the roots that own tile voxels.  The column's
+  //    run starts and the thread's halo pieces walk to their roots in
+  //    lockstep, one parent hop per round for all of them, so their LDS
+  //    latencies overlap; run members share their run start's root.  Roots
+  //    are fixed points, so a round may hop every slot; unused slots sit on
+  //    the node of box position (0, 0, 0), the smallest node in use, which is
+  //    a root (a parent never exceeds its node).  No lane writes par here, so
+  //    plain loads suffice; a path is shorter than the node count, which
+  //    bounds the rounds.
+  constexpr uint32_t ROOT0 = (uint32_t)G::template nid<QK>(0, 0, 0);
+  uint32_t fa[PT], fh[HV];
+#pragma unroll
+  for (int k = 0; k < PT; k++)
+    fa[k] = (fg(col[k]) && !(k > 0 && ((runs >> k) & 1u))) ? vnode(k) : ROOT0;
+#pragma unroll
+  for (int i = 0; i < HV; i++) fh[i] = ((hpiece >> i) & 1u) ? hnode(i) : ROOT0;
+  const bool wave_halo = __any(hpiece != 0u);  // waves without halo pieces skip those slots
+  bool more = true;
+#pragma unroll 1
+  for (int round = 0; more && round < G::template NIDS<QK>; round++) {
+    more = false;
+    // all loads of the round are issued before the first compare
+    uint32_t pa[PT];
+#pragma unroll
+    for (int k = 0; k < PT; k++) pa[k] = par[fa[k]];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+      more |= pa[k] != fa[k];
+      fa[k] = pa[k];
+    }
+    if (wave_halo) {
+      uint32_t ph[HV];
+#pragma unroll
+      for (int i = 0; i < HV; i++) ph[i] = par[fh[i]];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < HV; i++) {
+        more |= ph[i] != fh[i];
+        fh[i] = ph[i];
+      }
+    }
+  }
   uint32_t root[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++) {
     if (!fg(col[k])) root[k] = NONE;
     else if (k > 0 && ((runs >> k) & 1u)) root[k] = root[k - 1];
-    else root[k] = lfind(par, vnode(k));
+    else root[k] = fa[k];
   }
   uint32_t hroot[HV];
 #pragma unroll
-  for (int i = 0; i < HV; i++) hroot[i] = ((hpiece >> i) & 1u) ? lfind(par, hnode(i)) : NONE;
+  for (int i = 0; i < HV; i++) hroot[i] = ((hpiece >> i) & 1u) ? fh[i] : NONE;
 #pragma unroll
   for (int k = 0; k < PT; k++) {
     const uint32_t r = root[k];

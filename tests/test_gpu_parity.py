@@ -233,3 +233,14 @@ def test_voronoi_6conn_large_repeatable(gpu, shape, seed):
     assert N == expN and np.array_equal(got, exp), f"numpy input, run {run}"
     got, N = cc3d.connected_components(dvol, connectivity=6, out_dtype=np.uint32, return_N=True)
     assert N == expN and np.array_equal(got.cpu().numpy(), exp), f"device input, run {run}"
+
+
+@pytest.mark.parametrize("shape", [(5000, 5000, 3), (5504, 5504, 3)])
+def test_large_plane_staging(gpu, shape):
+  """uint64 volumes whose tile box spans about 1.8 GB (32-bit buffer offsets
+  near their limit) and about 2.2 GB (past it: the 64-bit address path)."""
+  import cc3d
+  import synth
+  vol = synth.voronoi(shape, 3000, seed=55, id_range=(1, None), dtype=np.uint64)
+  for conn in (26, 6):
+    _check(cc3d, vol, dict(connectivity=conn, out_dtype=np.uint32), f"largeplane{shape}")
