@@ -272,44 +272,54 @@ __device__ __host__ inline T from_bits(uint64_t b) {
 // ---------------------------------------------------------------------------
 // union-find
 // ---------------------------------------------------------------------------
-// LDS: parents are node ids; the root of a tree is its minimum node id and
-// every parent pointer points to a smaller id.
+// LDS: parents are 16-bit node ids (a tile box has fewer than 2^16 nodes);
+// the root of a tree is its minimum node id and every other node's parent is
+// a smaller id.
 //
-// lfind_h halves the path while other lanes are still unioning: each write is
-// an atomicMin to the grandparent, an ancestor in the same tree, so it can
-// never undo another lane's link (parents only ever decrease).
+// LDS has no 16-bit atomic min, so a union lowers a root's parent with a
+// compare-and-swap of the aligned 32-bit word holding it (pmin).  Path
+// halving (lfind_h) stores the grandparent with a plain 16-bit store: it only
+// ever writes a node that is no longer a root, and always an ancestor of it,
+// so trees stay trees and no link is lost (a union that loses a race retries
+// from the parent it displaced).
 __device__ inline uint32_t lload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ inline uint32_t lfind_h(uint32_t* par, uint32_t a) {
-  uint32_t p = lload(par + a);
+__device__ inline uint32_t pload(const uint16_t* par, uint32_t a) {
+  return __hip_atomic_load(par + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void pstore(uint16_t* par, uint32_t a, uint32_t v) {
+  __hip_atomic_store(par + a, (uint16_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// par[a] = min(par[a], v); returns the previous par[a]
+__device__ inline uint32_t pmin(uint16_t* par, uint32_t a, uint32_t v) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(par) + (a >> 1);
+  const uint32_t sh = (a & 1u) << 4;
+  uint32_t old = lload(w);
+  while (true) {
+    const uint32_t cur = (old >> sh) & 0xFFFFu;
+    if (cur <= v) return cur;
+    const uint32_t prev = atomicCAS(w, old, (old & ~(0xFFFFu << sh)) | (v << sh));
+    if (prev == old) return cur;
+    old = prev;
+  }
+}
+__device__ inline uint32_t lfind_h(uint16_t* par, uint32_t a) {
+  uint32_t p = pload(par, a);
   while (p != a) {
-    const uint32_t gp = lload(par + p);
+    const uint32_t gp = pload(par, p);
     if (gp == p) return p;
-    atomicMin(&par[a], gp);
+    pstore(par, a, gp);
     a = gp;
-    p = lload(par + a);
+    p = pload(par, a);
   }
   return a;
 }
 
-// Plain find for the read-only phase after all unions (path halving with
-// plain stores: every concurrent writer stores an ancestor).
-__device__ inline uint32_t lfind(uint32_t* par, uint32_t a) {
-  uint32_t p = lload(par + a);
-  while (p != a) {
-    const uint32_t gp = lload(par + p);
-    if (gp != p) __hip_atomic_store(par + a, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    a = p;
-    p = gp;
-  }
-  return a;
-}
-
-// Lock-free union: link the larger root under the smaller with atomicMin and
-// retry from the displaced parent when another lane got there first.  Returns
-// a node of the merged set (a good starting point for the next union of a).
-__device__ inline uint32_t lunion(uint32_t* par, uint32_t a, uint32_t b) {
+// Lock-free union: link the larger root under the smaller and retry from the
+// displaced parent when another lane got there first.  Returns a node of the
+// merged set (a good starting point for the next union of a).
+__device__ inline uint32_t lunion(uint16_t* par, uint32_t a, uint32_t b) {
   while (true) {
     a = lfind_h(par, a);
     b = lfind_h(par, b);
@@ -319,7 +329,7 @@ __device__ inline uint32_t lunion(uint32_t* par, uint32_t a, uint32_t b) {
       a = b;
       b = t;
     }
-    const uint32_t old = atomicMin(&par[a], b);
+    const uint32_t old = pmin(par, a, b);
     if (old == a) return b;
     a = old;
   }

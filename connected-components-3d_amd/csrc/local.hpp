@@ -181,7 +181,7 @@ __device__ inline T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 // its link to neighbour 0 is already in place.  Returns 0 exactly when p has
 // no linked backward neighbour (the reference gives p a new provisional label).
 template <class T, int MODE, int NB, int QK, class G, bool HALO>
-__device__ inline int link_voxel(const T* sv, uint32_t* par, int b, uint32_t pn, int bx, int by, int bz,
+__device__ inline int link_voxel(const T* sv, uint16_t* par, int b, uint32_t pn, int bx, int by, int bz,
                                  T cur, const Border& bd, const uint16_t* ut, uint32_t elig, bool col, T delta,
                                  T gmin, T gmax, bool row0) {
   using NH = Nbhd<NB>;
@@ -346,12 +346,14 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
   else return col_covers_all;
 }
 
-// Waves per SIMD the local kernel is compiled for: the LDS of one tile
-// (about 31 KB for 4-byte voxels) admits 5 workgroups of 4 waves per CU, so
-// registers are held to what 5 waves per SIMD allow.
-#ifndef LOCAL_WAVES_PER_EU
-#define LOCAL_WAVES_PER_EU 5
-#endif
+// Waves per SIMD the local kernel is compiled for, which caps its registers:
+// what the LDS of one tile admits (about 22 KB for 4-byte voxels: 7
+// workgroups of 4 waves per CU; 35 KB for 8-byte voxels: 4; 1- and 2-byte
+// voxels reach the limit of 8 waves per SIMD).
+template <class T>
+__host__ __device__ constexpr int local_waves() {
+  return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? 7 : 4;
+}
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
 // several connectivities at run time allocate it once).
@@ -360,14 +362,19 @@ struct LocalLds {
   static constexpr int NW = WG / 64;
   static constexpr int RW = (G::NIDS_MAX + 31) / 32;  // words of the root bitmask
   T sv[G::BOXN];
-  uint32_t par[G::NIDS_MAX];
-  uint16_t queue[G::TV];
-  uint16_t hqueue[G::IS3D ? G::HALO : 1];
-  uint32_t rbits[RW];    // local roots that own tile voxels
-  uint32_t rprefix[RW];  // exclusive popcount prefix of rbits
+  alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
+  // storage of steps that are never live together, separated by barriers
+  union {
+    uint16_t hqueue[G::IS3D ? G::HALO : 1];  // steps 1-2: halo run starts
+    uint16_t queue[G::TV];                   // steps 3-4: voxels needing unions
+    struct {                                 // steps 6-9
+      uint32_t rprefix[RW];                  // exclusive popcount prefix of rbits
+      uint32_t red_epl[NW], red_prov[NW];
+      int32_t red_first[NW], red_last[NW];
+    } late;
+  };
+  uint32_t rbits[RW];  // local roots that own tile voxels
   uint32_t qcount, nreq, hcount, nroots;
-  unsigned long long red_epl[NW], red_prov[NW];
-  long long red_first[NW], red_last[NW];
 };
 
 // Labels tile `tile` of the volume described by p.  Per-tile outputs go to
@@ -398,7 +405,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr uint32_t NONE = 0xFFFFFFFFu;
 
   T* sv = L.sv;
-  uint32_t* par = L.par;
+  uint16_t* par = L.par;
   // union table of this connectivity, class 0 (transitive connectivities only)
   constexpr bool UT = MODE != M_CONT && (NB == 26 || NB == 18 || NB == 8) && QK != Q_CONT8;
   const uint16_t* ut0 = UT ? p.utab + UTab<UT ? NB : 26>::OFF : nullptr;
@@ -693,6 +700,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   uint32_t my_epl = 0, my_prov = 0;
   int my_first = 0x7FFFFFFF, my_last = -1;
   uint32_t needA = 0, needB = 0;
+  uint32_t ia = 0, ib = 0;  // this thread's first slots at the front (A) and back (B) of the queue
   {
     const bool xlo = lx == 0, xhi = lx == G::TX - 1;
     // the column link of voxel 0 leaves the tile (3D: the z-1 halo plane; 2D:
@@ -731,16 +739,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     uint32_t base = 0;
     if (lane == 63) base = atomicAdd(&qcount, incl);
     base = __shfl(base, 63);
-    uint32_t ia = (base & 0xFFFFu) + ((incl - cnt) & 0xFFFFu);
-    uint32_t ib = (base >> 16) + ((incl - cnt) >> 16);
-#pragma unroll
-    for (int k = 0; k < PT; k++) {
-      int vx, vy, vz;
-      G::voxel(t, k, vx, vy, vz);
-      const uint16_t ps = (uint16_t)G::pos(vx, vy, vz);
-      if ((needA >> k) & 1u) queue[ia++] = ps;
-      if ((needB >> k) & 1u) queue[G::TV - 1 - ib++] = ps;
-    }
+    ia = (base & 0xFFFFu) + ((incl - cnt) & 0xFFFFu);
+    ib = (base >> 16) + ((incl - cnt) >> 16);
   }
   if constexpr (HRUNS) {
     // unions of the halo run start prepared in step 2, then any further ones
@@ -762,6 +762,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
   }
   __syncthreads();
+
+  // the queue shares LDS with hqueue, whose last reads are above
+#pragma unroll
+  for (int k = 0; k < PT; k++) {
+    int vx, vy, vz;
+    G::voxel(t, k, vx, vy, vz);
+    const uint16_t ps = (uint16_t)G::pos(vx, vy, vz);
+    if ((needA >> k) & 1u) queue[ia++] = ps;
+    if ((needB >> k) & 1u) queue[G::TV - 1 - ib++] = ps;
+  }
 
   // halo pieces are the roots left by step 2 (without pieces every
   // foreground halo voxel stands for itself).  Entry i of this thread: halo
@@ -925,14 +935,14 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       if (lane >= o) incl += y;
     }
     const uint32_t ex = incl - c0 - c1;
-    if (w0 < RW) L.rprefix[w0] = ex;
-    if (w0 + 1 < RW) L.rprefix[w0 + 1] = ex + c0;
+    if (w0 < RW) L.late.rprefix[w0] = ex;
+    if (w0 + 1 < RW) L.late.rprefix[w0 + 1] = ex + c0;
     if (lane == 63) L.nroots = incl;
   }
   __syncthreads();
   const uint32_t R = L.nroots;
   auto rank = [&](uint32_t r) -> uint32_t {
-    return L.rprefix[r >> 5] + (uint32_t)__popc(L.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
+    return L.late.rprefix[r >> 5] + (uint32_t)__popc(L.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
   };
   uint32_t* RLt = p.RL + slot * G::TV;
   // Keys are below 2^32, so they can be formed in 32-bit arithmetic: the
@@ -941,7 +951,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   const uint32_t kbox0 = (uint32_t)(x0 - 1) + (uint32_t)sx * ((uint32_t)(y0 - 1) + (uint32_t)sy * (uint32_t)(z0 - 1));
   for (int w = t; w < RW; w += WG) {
     uint32_t bits = L.rbits[w];
-    uint32_t rk = L.rprefix[w];
+    uint32_t rk = L.late.rprefix[w];
     while (bits) {
       const int bpos = __builtin_ctz(bits);
       bits &= bits - 1u;
@@ -1001,10 +1011,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     my_last = l > my_last ? l : my_last;
   }
   if (lane == 0) {
-    L.red_epl[wv] = my_epl;
-    L.red_prov[wv] = my_prov;
-    L.red_first[wv] = my_first;
-    L.red_last[wv] = my_last;
+    L.late.red_epl[wv] = my_epl;
+    L.late.red_prov[wv] = my_prov;
+    L.late.red_first[wv] = my_first;
+    L.late.red_last[wv] = my_last;
   }
   __syncthreads();
   if (t == 0) {
@@ -1012,10 +1022,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     long long f = 0x7FFFFFFFFFFFFFFFll, l = -1;
     int fl = 0x7FFFFFFF, ll = -1;
     for (int w = 0; w < NW; w++) {
-      e += L.red_epl[w];
-      pv2 += L.red_prov[w];
-      fl = (int)L.red_first[w] < fl ? (int)L.red_first[w] : fl;
-      ll = (int)L.red_last[w] > ll ? (int)L.red_last[w] : ll;
+      e += L.late.red_epl[w];
+      pv2 += L.late.red_prov[w];
+      fl = (int)L.late.red_first[w] < fl ? (int)L.late.red_first[w] : fl;
+      ll = (int)L.late.red_last[w] > ll ? (int)L.late.red_last[w] : ll;
     }
     if (ll >= 0) {  // tile-local rows to global rows
       f = (y0 + fl % G::TY) + sy * (IS3D ? z0 + fl / G::TY : 0);
@@ -1033,7 +1043,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 }
 
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(WG, LOCAL_WAVES_PER_EU) void k_local(LocalParams p) {
+__global__ __launch_bounds__(WG, local_waves<T>()) void k_local(LocalParams p) {
   __shared__ LocalLds<T, G> L;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
@@ -1043,7 +1053,7 @@ __global__ __launch_bounds__(WG, LOCAL_WAVES_PER_EU) void k_local(LocalParams p)
 // workgroup finds its volume and runs that volume's connectivity.  p carries
 // the shared per-tile arrays; its geometry fields are taken from the table.
 template <class T, int MODE, class G>
-__global__ __launch_bounds__(WG, LOCAL_WAVES_PER_EU) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
+__global__ __launch_bounds__(WG, local_waves<T>()) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
                                                     int nvol) {
   __shared__ LocalLds<T, G> L;
   const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
