@@ -485,7 +485,7 @@ __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
 template <class OUT, class G>
 __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_t tile, const VolGeo& v,
                                      OUT* __restrict__ out, uint32_t* lds_labels) {
-  static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
+  static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
   const int t = threadIdx.x;
   const int64_t tz = tile / (v.ntx * v.nty);
   const int64_t tr = tile - tz * v.ntx * v.nty;
@@ -507,15 +507,12 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
   lds_labels[0] = 0u;  // index 0: background
   for (uint32_t i = t; i < R; i += WG) lds_labels[i + 1] = LT[i];
   __syncthreads();
-  int lx, ly, lz;
-  G::voxel(t, 0, lx, ly, lz);
-  const int64_t gx = x0 + lx;
-  if (gx >= v.sx) return;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    G::voxel(t, k, lx, ly, lz);
-    const int64_t gy = y0 + ly, gz = z0 + lz;
-    if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lds_labels[j[k]];
+    int lx, ly, lz;
+    G::entry_voxel(8 * t + k, lx, ly, lz);
+    const int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lds_labels[j[k]];
   }
 }
 
@@ -526,7 +523,7 @@ constexpr int FIN_TILES = 2;
 
 template <class OUT, class G>
 __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
-  static_assert(G::PER_THREAD == 8, "one 8-voxel column per thread");
+  static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
   __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
   const int t = threadIdx.x;
   const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
@@ -553,8 +550,6 @@ __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict
     for (uint32_t i = t; i < R; i += WG) lab[n][i + 1] = LT[i];
   }
   __syncthreads();
-  int lx, ly, lz;
-  G::voxel(t, 0, lx, ly, lz);
 #pragma unroll
   for (int n = 0; n < FIN_TILES; n++) {
     const int64_t tile = first + n;
@@ -562,14 +557,12 @@ __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict
     const int64_t tz = tile / (v.ntx * v.nty);
     const int64_t tr = tile - tz * v.ntx * v.nty;
     const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
-    const int64_t gx = tx * G::TX + lx;
-    if (gx >= v.sx) continue;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       int vx, vy, vz;
-      G::voxel(t, k, vx, vy, vz);
-      const int64_t gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
-      if (gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[n][j[n][k]];
+      G::entry_voxel(8 * t + k, vx, vy, vz);
+      const int64_t gx = tx * G::TX + vx, gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
+      if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[n][j[n][k]];
     }
   }
 }

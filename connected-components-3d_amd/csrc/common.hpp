@@ -66,15 +66,17 @@ __device__ inline int find_vol(const int64_t* vtile0, int n, int64_t slot) {
   return lo;
 }
 
-// Tile geometry.  A tile is TX x TY x TZ voxels; the workgroup stages it in LDS
-// together with a one-voxel halo on the sides that backward neighbours reach
-// (x-1..x+1, y-1..y+1 and z-1 in 3D; x-1..x+1 and y-1 in 2D).
-template <int TX_, int TY_, int TZ_, bool IS3D_>
+// Tile geometry.  A tile is TX x TY x TZ voxels; the local kernel's
+// workgroup of NT threads (one column of PER_THREAD voxels each) stages it in
+// LDS together with a one-voxel halo on the sides that backward neighbours
+// reach (x-1..x+1, y-1..y+1 and z-1 in 3D; x-1..x+1 and y-1 in 2D).
+template <int TX_, int TY_, int TZ_, bool IS3D_, int NT_>
 struct TileGeo {
   static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
   static constexpr bool IS3D = IS3D_;
+  static constexpr int NT = NT_;
   static constexpr int TV = TX * TY * TZ;
-  static constexpr int PER_THREAD = TV / WG;
+  static constexpr int PER_THREAD = TV / NT;
   static constexpr int BX = TX + 2;
   static constexpr int BY = IS3D ? TY + 2 : TY + 1;
   static constexpr int BZ = IS3D ? TZ + 1 : 1;
@@ -82,9 +84,13 @@ struct TileGeo {
   static constexpr int RING = 2 * BX + 2 * (BY - 2);
   static constexpr int HALO = IS3D ? BX * BY + TZ * RING : BX + 2 * TY;
   static constexpr int NN = TV + HALO;
-  static_assert(TV % WG == 0, "tile must be a multiple of the workgroup");
-  static_assert(IS3D ? (TX * TY == WG && TZ == PER_THREAD) : (TX * TY == WG * PER_THREAD && TX <= WG),
+  static_assert(TV % NT == 0 && NT % 64 == 0, "tile must be a multiple of the workgroup");
+  static_assert(IS3D ? (TX * TY == NT && TZ == PER_THREAD) : (TX * TY == NT * PER_THREAD && TX <= NT),
                 "one column of PER_THREAD voxels per thread");
+  // J entry e (thread-major: column e / PER_THREAD, voxel e % PER_THREAD)
+  __device__ static inline void entry_voxel(int e, int& lx, int& ly, int& lz) {
+    voxel(e / PER_THREAD, e % PER_THREAD, lx, ly, lz);
+  }
   static_assert((TX & (TX - 1)) == 0 && (TY & (TY - 1)) == 0 && (TZ & (TZ - 1)) == 0,
                 "tile sides are powers of two");
 
@@ -219,8 +225,11 @@ struct TileGeo {
   }
 };
 
-using Geo3 = TileGeo<32, 8, 8, true>;
-using Geo2 = TileGeo<64, 32, 1, false>;
+// 3D: 32 x 8 x 8 voxels; 2D: 64 x 32 pixels; 256 threads.  (A 32 x 16 x 4
+// tile with 512 threads puts fewer columns on the tile border but holds
+// fewer tiles per CU in flight: measured slower.)
+using Geo3 = TileGeo<32, 8, 8, true, 256>;
+using Geo2 = TileGeo<64, 32, 1, false, 256>;
 
 // Global key of a voxel.
 template <int QK>

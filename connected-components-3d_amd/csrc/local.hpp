@@ -347,10 +347,10 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
 }
 
 // Waves per SIMD the local kernel is compiled for, which caps its registers:
-// what the LDS of one tile admits (about 22 KB for 4-byte voxels: 7
-// workgroups of 4 waves per CU; 35 KB for 8-byte voxels: 4; 1- and 2-byte
-// voxels reach the limit of 8 waves per SIMD).
-template <class T>
+// what the LDS of one tile admits (22.9 KB for 4-byte voxels: 7 workgroups
+// of 4 waves per CU; 35 KB for 8-byte voxels: 4; 1- and 2-byte voxels reach
+// the limit of 8 waves per SIMD).
+template <class T, class G>
 __host__ __device__ constexpr int local_waves() {
   return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? 7 : 4;
 }
@@ -359,7 +359,7 @@ __host__ __device__ constexpr int local_waves() {
 // several connectivities at run time allocate it once).
 template <class T, class G>
 struct LocalLds {
-  static constexpr int NW = WG / 64;
+  static constexpr int NW = G::NT / 64;
   static constexpr int RW = (G::NIDS_MAX + 31) / 32;  // words of the root bitmask
   T sv[G::BOXN];
   alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
@@ -391,9 +391,10 @@ struct LocalLds {
 template <class T, int MODE, int NB, int QK, class G>
 __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t slot, uint32_t keybase,
                                   LocalLds<T, G>& L) {
-  constexpr int NW = WG / 64;
+  constexpr int NT = G::NT;  // threads of the workgroup, one column each
+  constexpr int NW = NT / 64;
   constexpr int PT = G::PER_THREAD;
-  constexpr int HPT = (G::HALO + WG - 1) / WG;
+  constexpr int HPT = (G::HALO + NT - 1) / NT;
   constexpr bool PIECES = !(QK == Q_BIN4 || QK == Q_CONT8);
   constexpr bool IS3D = G::IS3D;
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
@@ -435,7 +436,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     nreq = 0;
     hcount = 0;
   }
-  for (int w = t; w < RW; w += WG) L.rbits[w] = 0;
+  for (int w = t; w < RW; w += NT) L.rbits[w] = 0;
   __syncthreads();
 
   // this thread's column
@@ -454,19 +455,22 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   //    volume read as background.  Column runs get their parent here.  All
   //    loads (column, its predecessor, halo) are issued before any value is
   //    used, so one wait covers them.
-  constexpr int HLEN = G::TZ > G::BY ? G::TZ : G::BY;
+  // HRUNS bottom plane: columns along y cut into NBS segments of BSEG voxels
+  constexpr int NBS = (G::BY + 9) / 10;
+  constexpr int BSEG = (G::BY + NBS - 1) / NBS;
+  constexpr int HLEN = G::TZ > BSEG ? G::TZ : BSEG;
   constexpr int HV = HRUNS ? HLEN : HPT;
   T col[PT];
   T hv[HV];
   T pv;
   uint32_t runs = 0;  // bit k: voxel k continues the run of its predecessor
-  // 3D halo as columns (HRUNS): ring position r (threads 0..RING-1) along z,
-  // bottom plane column bx (threads RING..RING+BX-1) along y.  Otherwise
-  // thread t stages halo voxels t, t + WG, ...
+  // 3D halo as columns (HRUNS): ring position r (threads 0..RING-1) along z;
+  // bottom plane column bx, segment g (thread RING + bx + BX * g) along y
+  // from y = g * BSEG.  Otherwise thread t stages halo voxels t, t + NT, ...
   constexpr int RING = G::RING;
-  const bool hcol = HRUNS && t < RING + G::BX;
+  const bool hcol = HRUNS && t < RING + G::BX * NBS;
   const bool ring = t < RING;
-  int cbx = 0, cby = 0;
+  int cbx = 0, cby = 0, hby0 = 0;  // hby0: first y of a bottom segment
   if (HRUNS) {
     if (ring) {
       if (t < G::BX) {
@@ -480,11 +484,13 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         cby = ((t - 2 * G::BX) >> 1) + 1;
       }
     } else {
-      cbx = t - RING;
+      cbx = (t - RING) % G::BX;
       cby = 0;
+      if constexpr (NBS > 1) hby0 = ((t - RING) / G::BX) * BSEG;
     }
   }
-  const int hlen = ring ? G::TZ : G::BY;
+  const int hlen = ring ? G::TZ : (G::BY - hby0 < BSEG ? G::BY - hby0 : BSEG);
+  T hpv = T(0);  // bottom segment after the first: the voxel before it
 #pragma unroll
   for (int k = 0; k < HV; k++) hv[k] = T(0);
   // The box (tile plus halo) spans fewer than 2^31 bytes of the volume for
@@ -512,23 +518,27 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     pv = bload<T>(rs, pok ? c0 - CST : OOB);
     if constexpr (HRUNS) {
       if (hcol) {
-        // ring column: box (cbx, cby, 1 + k); bottom column: box (cbx, k, 0)
-        const int64_t hx = x0 + cbx - 1, hy = y0 + cby - 1;
+        // ring column: box (cbx, cby, 1 + k); bottom segment: box (cbx, hby0 + k, 0)
+        const int64_t hx = x0 + cbx - 1, hy = y0 + cby - 1, hyb = y0 - 1 + hby0;
         const bool fixok = hx >= 0 && hx < sx && (ring ? (hy >= 0 && hy < sy) : z0 > 0);
-        const uint32_t h0 = (uint32_t)cbx * SZ + (ring ? (uint32_t)cby * RB + PB : 0u);
+        const uint32_t h0 = (uint32_t)cbx * SZ + (ring ? (uint32_t)cby * RB + PB : (uint32_t)hby0 * RB);
         const uint32_t hst = ring ? PB : RB;
 #pragma unroll
         for (int k = 0; k < HV; k++) {
-          const bool along = ring ? (k < G::TZ) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
+          const bool along = ring ? (k < G::TZ) : (k < hlen && hyb + k >= 0 && hyb + k < sy);
           hv[k] = bload<T>(rs, (fixok && along) ? h0 + (uint32_t)k * hst : OOB);
+        }
+        if constexpr (NBS > 1) {
+          const bool pbok = !ring && hby0 > 0 && fixok && hyb - 1 >= 0 && hyb - 1 < sy;
+          hpv = bload<T>(rs, pbok ? h0 - RB : OOB);
         }
       }
     } else {
 #pragma unroll
       for (int k = 0; k < HV; k++) {
-        const bool live = t + WG * k < G::HALO;
+        const bool live = t + NT * k < G::HALO;
         int bx, by, bz;
-        G::halo_coords(live ? t + WG * k : 0, bx, by, bz);
+        G::halo_coords(live ? t + NT * k : 0, bx, by, bz);
         const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
         const bool ok = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0;
         hv[k] = bload<T>(rs, ok ? (uint32_t)bx * SZ + (uint32_t)by * RB + (uint32_t)bz * PB : OOB);
@@ -558,7 +568,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     if constexpr (HRUNS) {
       if (hcol) {  // waves holding no halo column skip the address work
         const int64_t hx = x0 + cbx - 1;
-        const int64_t hy0 = ring ? y0 + cby - 1 : y0 - 1, hz0 = ring ? z0 : z0 - 1;
+        const int64_t hy0 = ring ? y0 + cby - 1 : y0 - 1 + hby0, hz0 = ring ? z0 : z0 - 1;
         const bool fixok = hx >= 0 && hx < sx && (ring ? (hy0 >= 0 && hy0 < sy) : (hz0 >= 0 && hz0 < sz));
         const int64_t hstride = ring ? sx * sy : sx;
         // the base may lie one row before the volume (y0 = 0); only addresses
@@ -566,18 +576,23 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         const T* hbase = in + (fixok ? hx + sx * (hy0 + sy * hz0) : 0);
 #pragma unroll
         for (int k = 0; k < HV; k++) {
-          const bool along = ring ? (k < G::TZ && z0 + k < sz) : (y0 - 1 + k >= 0 && y0 - 1 + k < sy);
+          const bool along = ring ? (k < G::TZ && z0 + k < sz) : (k < hlen && hy0 + k >= 0 && hy0 + k < sy);
           hok[k] = fixok && along;
           hv[k] = *(hok[k] ? hbase + k * hstride : in);
+        }
+        if constexpr (NBS > 1) {
+          const bool pbok = !ring && hby0 > 0 && fixok && hy0 - 1 >= 0 && hy0 - 1 < sy;
+          hpv = *(pbok ? hbase - sx : in);
+          hpv = pbok ? hpv : T(0);
         }
       }
     } else {
 #pragma unroll
       for (int k = 0; k < HV; k++) {
         int bx, by, bz;
-        G::halo_coords(t + WG * k < G::HALO ? t + WG * k : 0, bx, by, bz);
+        G::halo_coords(t + NT * k < G::HALO ? t + NT * k : 0, bx, by, bz);
         const int64_t hx = x0 + bx - 1, hy = y0 + by - 1, hz = IS3D ? z0 + bz - 1 : 0;
-        const bool live = t + WG * k < G::HALO;
+        const bool live = t + NT * k < G::HALO;
         hok[k] = live && hx >= 0 && hx < sx && hy >= 0 && hy < sy && hz >= 0 && hz < sz;
         hv[k] = in[hok[k] ? hx + sx * (hy + sy * hz) : 0];
       }
@@ -611,16 +626,20 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     uint32_t hneed = 0;  // bit k: halo voxel k of this column is a run start
     if (hcol) {
       const int cpos = ring ? 0 : 1;  // E or K
-      T prev = T(0);
-      uint32_t start = 0;
+      // a bottom segment after the first continues the run of the voxel
+      // before it (staged by the previous segment's thread)
+      T prev = (NBS > 1 && !ring) ? hpv : T(0);
+      uint32_t start = (NBS > 1 && !ring) ? (uint32_t)G::box(cbx, hby0 > 0 ? hby0 - 1 : 0, 0) : 0u;
 #pragma unroll
       for (int k = 0; k < HLEN; k++) {
         if (k >= hlen) break;
-        const int bx = cbx, by = ring ? cby : k, bz = ring ? k + 1 : 0;
+        const int bx = cbx, by = ring ? cby : hby0 + k, bz = ring ? k + 1 : 0;
         const int b = G::box(bx, by, bz);
         const T v = hv[k];
         sv[b] = v;
-        const bool linkable = ring ? k >= 1 : (k >= 2 && k <= G::TY);
+        // predecessor in the same neighbouring tile: along z for the ring,
+        // rows 1..TY of the bottom plane (the tile below) along y
+        const bool linkable = ring ? k >= 1 : (by >= 2 && by <= G::TY);
         bool member = false;
         if (fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax)) {
           const uint32_t el = halo_elig<G>(bx, by, bz);
@@ -648,12 +667,12 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 #pragma unroll
       for (int k = 0; k < HLEN; k++)
         if ((hneed >> k) & 1u)
-          hqueue[hbase++] = (uint16_t)(ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * k);
+          hqueue[hbase++] = (uint16_t)(ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * (hby0 + k));
     }
   } else {
 #pragma unroll
     for (int k = 0; k < HV; k++) {
-      const int h = t + WG * k;
+      const int h = t + NT * k;
       if (h < G::HALO) {
         int bx, by, bz;
         G::halo_coords(h, bx, by, bz);
@@ -682,7 +701,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       h_u = ut0[lk];  // class 0: halo pairs of one neighbouring tile are all unioned here
     }
   } else if constexpr (PIECES) {
-    for (int h = t; h < G::HALO; h += WG) {
+    for (int h = t; h < G::HALO; h += NT) {
       int bx, by, bz;
       G::halo_coords(h, bx, by, bz);
       const int b = G::box(bx, by, bz);
@@ -753,7 +772,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       }
     }
 #pragma unroll 1
-    for (uint32_t q = t + WG; q < h_nh; q += WG) {
+    for (uint32_t q = t + NT; q < h_nh; q += NT) {
       int bx, by, bz;
       G::halo_coords(hqueue[q], bx, by, bz);
       const int b = G::box(bx, by, bz);
@@ -775,16 +794,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 
   // halo pieces are the roots left by step 2 (without pieces every
   // foreground halo voxel stands for itself).  Entry i of this thread: halo
-  // column voxel i under HRUNS, else halo voxel t + WG * i.
+  // column voxel i under HRUNS, else halo voxel t + NT * i.
   auto hnode = [&](int i) -> uint32_t {
-    if (HRUNS) return (uint32_t)G::box(cbx, ring ? cby : i, ring ? i + 1 : 0);
+    if (HRUNS) return (uint32_t)G::box(cbx, ring ? cby : hby0 + i, ring ? i + 1 : 0);
     int bx, by, bz;
-    G::halo_coords(t + WG * i, bx, by, bz);
+    G::halo_coords(t + NT * i, bx, by, bz);
     return (uint32_t)G::template nid<QK>(bx, by, bz);
   };
   auto hindex = [&](int i) -> uint32_t {
-    if (HRUNS) return (uint32_t)(ring ? G::BX * G::BY + i * RING + t : cbx + G::BX * i);
-    return (uint32_t)(t + WG * i);
+    if (HRUNS) return (uint32_t)(ring ? G::BX * G::BY + i * RING + t : cbx + G::BX * (hby0 + i));
+    return (uint32_t)(t + NT * i);
   };
   uint32_t hpiece = 0;
 #pragma unroll
@@ -832,8 +851,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     // Two entries per round: the second one's table load is issued first and
     // the first one's right after, so both latencies overlap.
 #pragma unroll 1
-    for (uint32_t q = t; q < nQ; q += 2 * WG) {
-      const uint32_t q2 = q + WG;
+    for (uint32_t q = t; q < nQ; q += 2 * NT) {
+      const uint32_t q2 = q + NT;
       Item i2;  // left unset past the end
       if (q2 < nQ) i2 = prep(q2);
       const Item i1 = prep(q);
@@ -842,7 +861,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
   } else {
 #pragma unroll 1
-    for (uint32_t q = t; q < nQ; q += WG) {
+    for (uint32_t q = t; q < nQ; q += NT) {
       const bool colq = q >= nA;
       const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
       const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
@@ -949,7 +968,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   // wrap-around of the intermediate terms cancels.  Box position (bx, by, bz)
   // is voxel (x0 + bx - 1, y0 + by - 1, z0 + bz - 1).
   const uint32_t kbox0 = (uint32_t)(x0 - 1) + (uint32_t)sx * ((uint32_t)(y0 - 1) + (uint32_t)sy * (uint32_t)(z0 - 1));
-  for (int w = t; w < RW; w += WG) {
+  for (int w = t; w < RW; w += NT) {
     uint32_t bits = L.rbits[w];
     uint32_t rk = L.late.rprefix[w];
     while (bits) {
@@ -970,35 +989,51 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
 
   // 7. per-voxel root index, thread-major: one 8- or 16-byte store per thread
-  static_assert(PT == 8, "J stores assume 8 voxels per thread");
+  static_assert(PT == 8 || PT == 4, "J stores take 4 or 8 voxels per thread");
   uint32_t jv[PT];
 #pragma unroll
   for (int k = 0; k < PT; k++)
     jv[k] = root[k] == NONE ? 0u : (k > 0 && root[k] == root[k - 1]) ? jv[k > 0 ? k - 1 : 0] : rank(root[k]) + 1u;
-  if (R <= 255) {
-    uint64_t w = 0;
+  uint32_t jw[PT / 2];  // 16-bit entries, two per word
 #pragma unroll
-    for (int k = 0; k < PT; k++) w |= (uint64_t)jv[k] << (8 * k);
-    reinterpret_cast<uint64_t*>(p.J8 + slot * G::TV)[t] = w;
+  for (int i = 0; i < PT / 2; i++) jw[i] = jv[2 * i] | (jv[2 * i + 1] << 16);
+  if (R <= 255) {
+    uint32_t w8[PT / 4];  // 8-bit entries, four per word
+#pragma unroll
+    for (int i = 0; i < PT / 4; i++) w8[i] = (jw[2 * i] & 0xFFu) | ((jw[2 * i] >> 8) & 0xFF00u) |
+                                             ((jw[2 * i + 1] & 0xFFu) << 16) | ((jw[2 * i + 1] >> 16) << 24);
+    if constexpr (PT == 8) reinterpret_cast<uint2*>(p.J8 + slot * G::TV)[t] = make_uint2(w8[0], w8[PT / 4 - 1]);
+    else reinterpret_cast<uint32_t*>(p.J8 + slot * G::TV)[t] = w8[0];
   } else {
-    uint4 w;
-    w.x = jv[0] | (jv[1] << 16);
-    w.y = jv[2] | (jv[3] << 16);
-    w.z = jv[4] | (jv[5] << 16);
-    w.w = jv[6] | (jv[7] << 16);
-    reinterpret_cast<uint4*>(p.J16 + slot * G::TV)[t] = w;
+    if constexpr (PT == 8)
+      reinterpret_cast<uint4*>(p.J16 + slot * G::TV)[t] = make_uint4(jw[0], jw[1], jw[PT / 2 - 2], jw[PT / 2 - 1]);
+    else reinterpret_cast<uint2*>(p.J16 + slot * G::TV)[t] = make_uint2(jw[0], jw[PT / 2 - 1]);
   }
 
   // 8. cross-tile requests: one per halo piece whose tree owns tile voxels,
   //    as (root node, halo index)
-  uint32_t* REQt = p.REQ + slot * G::HALO;
+  //    slots from a wave-level scan and one LDS atomic per wave
+  uint32_t rq = 0;  // bit i: halo entry i makes a request
 #pragma unroll
   for (int i = 0; i < HV; i++) {
     const uint32_t r = hroot[i];
-    if (r != NONE && ((L.rbits[r >> 5] >> (r & 31)) & 1u)) {
-      const uint32_t q = atomicAdd(&nreq, 1u);
-      REQt[q] = (r << 16) | hindex(i);
+    if (r != NONE && ((L.rbits[r >> 5] >> (r & 31)) & 1u)) rq |= 1u << i;
+  }
+  if (__any(rq != 0u)) {
+    const uint32_t rc = (uint32_t)__popc(rq);
+    uint32_t rincl = rc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(rincl, o);
+      if (lane >= o) rincl += y;
     }
+    uint32_t rbase = 0;
+    if (lane == 63) rbase = atomicAdd(&nreq, rincl);
+    uint32_t q = __shfl(rbase, 63) + rincl - rc;
+    uint32_t* REQt = p.REQ + slot * G::HALO;
+#pragma unroll
+    for (int i = 0; i < HV; i++)
+      if ((rq >> i) & 1u) REQt[q++] = (hroot[i] << 16) | hindex(i);
   }
 
   // 9. counters
@@ -1043,7 +1078,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 }
 
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(WG, local_waves<T>()) void k_local(LocalParams p) {
+__global__ __launch_bounds__(G::NT, (local_waves<T, G>())) void k_local(LocalParams p) {
   __shared__ LocalLds<T, G> L;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
@@ -1052,8 +1087,10 @@ __global__ __launch_bounds__(WG, local_waves<T>()) void k_local(LocalParams p) {
 // A batch of 3D volumes in one launch: grid = all tiles of all volumes; each
 // workgroup finds its volume and runs that volume's connectivity.  p carries
 // the shared per-tile arrays; its geometry fields are taken from the table.
+// Holding three connectivities' code, it is compiled for one wave per SIMD
+// fewer than k_local, which keeps its registers from spilling.
 template <class T, int MODE, class G>
-__global__ __launch_bounds__(WG, local_waves<T>()) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
+__global__ __launch_bounds__(G::NT, (local_waves<T, G>() - 1)) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
                                                     int nvol) {
   __shared__ LocalLds<T, G> L;
   const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
@@ -1086,7 +1123,7 @@ hipError_t launch_local_batch_binary(int dt, const LocalParams& p, const BatchVo
 template <int MODE>
 hipError_t launch_local_batch_mode(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
                                    int nvol, hipStream_t s) {
-  const dim3 grid((unsigned)p.ntiles), block(WG);
+  const dim3 grid((unsigned)p.ntiles), block(Geo3::NT);
   switch (dt) {
     case DT_U8: hipLaunchKernelGGL((k_local_batch<uint8_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
     case DT_U16: hipLaunchKernelGGL((k_local_batch<uint16_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
@@ -1101,9 +1138,9 @@ hipError_t launch_local_batch_mode(int dt, const LocalParams& p, const BatchVol*
 
 template <int MODE>
 hipError_t launch_local_mode(int dt, int nb, int qk, const LocalParams& p, hipStream_t s) {
-  const dim3 grid((unsigned)p.ntiles), block(WG);
+  const dim3 grid((unsigned)p.ntiles);
 #define CC3D_L(T, NB, QK, G) \
-  hipLaunchKernelGGL((k_local<T, MODE, NB, QK, G>), grid, block, 0, s, p)
+  hipLaunchKernelGGL((k_local<T, MODE, NB, QK, G>), grid, dim3(G::NT), 0, s, p)
 #define CC3D_NB(T)                                                               \
   switch (nb) {                                                                  \
     case 26: CC3D_L(T, 26, Q_NONE, Geo3); break;                                 \
