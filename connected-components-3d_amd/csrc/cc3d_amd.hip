@@ -201,6 +201,7 @@ __device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int
 // lanes idle.  The label pass does less work per root and runs best wider.
 constexpr int TILE_LANES = 8;
 constexpr int LABEL_LANES = 16;
+constexpr int GLB = 4;  // roots per lane whose label loads are batched in k_labels
 
 // Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests.
 template <class G, int QK>
@@ -467,12 +468,26 @@ __global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
     const uint32_t R = p.TR[slot];
     if (R == 0) continue;
     const uint32_t base = p.nvol ? p.prefix[vol_geo(p, slot).key0 >> 5] : 0u;
-    for (uint32_t j = sub; j < R; j += LABEL_LANES) {
-      const uint32_t g = p.RL[slot * G::TV + j];
-      const uint32_t r = p.GP[g];
-      const uint32_t wd = r >> 5;
-      const uint32_t below = p.bits[wd] & ((1u << (r & 31)) - 1u);
-      p.RL[slot * G::TV + j] = p.prefix[wd] - base + __popc(below) + 1u;
+    // GLB roots per lane at a time: each level's loads are issued together
+    for (uint32_t j0 = sub; j0 < R; j0 += LABEL_LANES * GLB) {
+      uint32_t g[GLB], r[GLB], bw[GLB], pf[GLB];
+#pragma unroll
+      for (int u = 0; u < GLB; u++) {
+        const uint32_t j = j0 + u * LABEL_LANES;
+        g[u] = p.RL[slot * G::TV + (j < R ? j : j0)];
+      }
+#pragma unroll
+      for (int u = 0; u < GLB; u++) r[u] = p.GP[g[u]];
+#pragma unroll
+      for (int u = 0; u < GLB; u++) {
+        bw[u] = p.bits[r[u] >> 5];
+        pf[u] = p.prefix[r[u] >> 5];
+      }
+#pragma unroll
+      for (int u = 0; u < GLB; u++) {
+        const uint32_t j = j0 + u * LABEL_LANES;
+        if (j < R) p.RL[slot * G::TV + j] = pf[u] - base + __popc(bw[u] & ((1u << (r[u] & 31)) - 1u)) + 1u;
+      }
     }
   }
 }

@@ -347,12 +347,20 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
 }
 
 // Waves per SIMD the local kernel is compiled for, which caps its registers:
-// what the LDS of one tile admits (22.9 KB for 4-byte voxels: 7 workgroups
-// of 4 waves per CU; 35 KB for 8-byte voxels: 4; 1- and 2-byte voxels reach
-// the limit of 8 waves per SIMD).
+// what the LDS of one tile admits (20.4 KB for 4-byte voxels: 8 workgroups
+// of 4 waves per CU, the limit of 8 waves per SIMD, as for 1- and 2-byte
+// voxels; 32 KB for 8-byte voxels: 5).
 template <class T, class G>
 __host__ __device__ constexpr int local_waves() {
-  return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? 7 : 4;
+  return sizeof(T) == 8 ? 5 : 8;
+}
+
+// Work-queue capacity: 42 % of a tile.  The tiles of the 512^3 Voronoi
+// workload queue 621 voxels on average and at most 847 of 2048; voxels past
+// the capacity are unioned by their own thread after the queue.
+template <class G>
+__host__ __device__ constexpr int queue_cap() {
+  return (G::TV * 107 / 256) & ~7;
 }
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
@@ -361,20 +369,24 @@ template <class T, class G>
 struct LocalLds {
   static constexpr int NW = G::NT / 64;
   static constexpr int RW = (G::NIDS_MAX + 31) / 32;  // words of the root bitmask
+  static constexpr int QC = queue_cap<G>();
   T sv[G::BOXN];
   alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
   // storage of steps that are never live together, separated by barriers
   union {
     uint16_t hqueue[G::IS3D ? G::HALO : 1];  // steps 1-2: halo run starts
-    uint16_t queue[G::TV];                   // steps 3-4: voxels needing unions
-    struct {                                 // steps 6-9
-      uint32_t rprefix[RW];                  // exclusive popcount prefix of rbits
+    struct {
+      uint16_t queue[QC];  // steps 3-4: voxel position | run member << 15
+      uint32_t rbits[RW];  // steps 5-8: local roots that own tile voxels
+    } w;
+    struct {               // steps 6-9, over the queue only
+      uint32_t rprefix[RW];  // exclusive popcount prefix of rbits
       uint32_t red_epl[NW], red_prov[NW];
       int32_t red_first[NW], red_last[NW];
     } late;
   };
-  uint32_t rbits[RW];  // local roots that own tile voxels
   uint32_t qcount, nreq, hcount, nroots;
+  static_assert(QC % 2 == 0 && RW * 4 + NW * 16 <= QC * 2, "the late storage stays clear of rbits");
 };
 
 // Labels tile `tile` of the volume described by p.  Per-tile outputs go to
@@ -410,7 +422,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   // union table of this connectivity, class 0 (transitive connectivities only)
   constexpr bool UT = MODE != M_CONT && (NB == 26 || NB == 18 || NB == 8) && QK != Q_CONT8;
   const uint16_t* ut0 = UT ? p.utab + UTab<UT ? NB : 26>::OFF : nullptr;
-  uint16_t* queue = L.queue;
+  uint16_t* queue = L.w.queue;
+  constexpr uint32_t QC = LocalLds<T, G>::QC;
   uint16_t* hqueue = L.hqueue;
   uint32_t& qcount = L.qcount;
   uint32_t& nreq = L.nreq;
@@ -436,7 +449,6 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     nreq = 0;
     hcount = 0;
   }
-  for (int w = t; w < RW; w += NT) L.rbits[w] = 0;
   __syncthreads();
 
   // this thread's column
@@ -713,13 +725,14 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
 
   // 3. counters over the column and the work queue: voxels still needing
-  //    neighbour unions (run starts first, run members from the back)
+  //    neighbour unions (run starts, and run members whose column link does
+  //    not cover their neighbourhood)
   // per-thread counts fit 32 bits; rows are tile-local (ly + TY * lz), whose
   // order is the order of the global rows y + sy * z
   uint32_t my_epl = 0, my_prov = 0;
   int my_first = 0x7FFFFFFF, my_last = -1;
-  uint32_t needA = 0, needB = 0;
-  uint32_t ia = 0, ib = 0;  // this thread's first slots at the front (A) and back (B) of the queue
+  uint32_t need = 0;  // bit k: voxel k is queued
+  uint32_t qs = 0;    // this thread's first queue slot
   {
     const bool xlo = lx == 0, xhi = lx == G::TX - 1;
     // the column link of voxel 0 leaves the tile (3D: the z-1 halo plane; 2D:
@@ -739,16 +752,13 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       const int row = vy + G::TY * vz;
       my_first = row < my_first ? row : my_first;
       my_last = row > my_last ? row : my_last;
-      if ((runs >> k) & 1u) {
-        if (!column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0, delta,
-                                          gmin, gmax))
-          needB |= 1u << k;
-      } else {
-        needA |= 1u << k;
-      }
+      if (!((runs >> k) & 1u) ||
+          !column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0, delta,
+                                        gmin, gmax))
+        need |= 1u << k;
     }
-    // wave-level exclusive scan of the packed counts, one LDS atomic per wave
-    const uint32_t cnt = (uint32_t)__popc(needA) | ((uint32_t)__popc(needB) << 16);
+    // wave-level exclusive scan of the counts, one LDS atomic per wave
+    const uint32_t cnt = (uint32_t)__popc(need);
     uint32_t incl = cnt;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -757,9 +767,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
     uint32_t base = 0;
     if (lane == 63) base = atomicAdd(&qcount, incl);
-    base = __shfl(base, 63);
-    ia = (base & 0xFFFFu) + ((incl - cnt) & 0xFFFFu);
-    ib = (base >> 16) + ((incl - cnt) >> 16);
+    qs = __shfl(base, 63) + incl - cnt;
   }
   if constexpr (HRUNS) {
     // unions of the halo run start prepared in step 2, then any further ones
@@ -782,15 +790,19 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
   __syncthreads();
 
-  // the queue shares LDS with hqueue, whose last reads are above
+  // The queue and the root bitmask share LDS with hqueue, whose last reads
+  // are above.  Voxels past the queue's capacity stay with their thread.
+  uint32_t over = 0;  // bit k: voxel k did not fit the queue
 #pragma unroll
   for (int k = 0; k < PT; k++) {
+    if (!((need >> k) & 1u)) continue;
     int vx, vy, vz;
     G::voxel(t, k, vx, vy, vz);
-    const uint16_t ps = (uint16_t)G::pos(vx, vy, vz);
-    if ((needA >> k) & 1u) queue[ia++] = ps;
-    if ((needB >> k) & 1u) queue[G::TV - 1 - ib++] = ps;
+    if (qs < QC) queue[qs] = (uint16_t)(G::pos(vx, vy, vz) | (((runs >> k) & 1u) << 15));
+    else over |= 1u << k;
+    qs++;
   }
+  for (int w = t; w < RW; w += NT) L.w.rbits[w] = 0;
 
   // halo pieces are the roots left by step 2 (without pieces every
   // foreground halo voxel stands for itself).  Entry i of this thread: halo
@@ -813,7 +825,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       if (!PIECES || par[n] == n) hpiece |= 1u << i;
     }
   }
-  const uint32_t nA = qcount & 0xFFFFu, nQ = nA + (qcount >> 16);
+  const uint32_t nQ = qcount < QC ? qcount : QC;
   __syncthreads();
 
   // 4. queued voxels
@@ -823,9 +835,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       uint32_t pn, lk, u, cm;
       int bx, by, bz;
     };
-    auto prep = [&](uint32_t q) -> Item {
-      const bool colq = q >= nA;
-      const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
+    auto prep_at = [&](int ps, bool colq) -> Item {
       const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
       Item it;
       it.bx = vx + 1;
@@ -838,6 +848,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       it.u = ut0[(cls << Nbhd<NB>::N) | it.lk];
       it.cm = colq ? ~1u : ~0u;  // a run member's link to neighbour 0 is in place
       return it;
+    };
+    auto prep = [&](uint32_t q) -> Item {
+      const uint32_t e = queue[q];
+      return prep_at((int)(e & 0x7FFFu), (e >> 15) != 0u);
     };
     auto unions = [&](const Item& it) {
       uint32_t a = it.pn, u = it.u & it.cm;
@@ -859,11 +873,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       unions(i1);
       if (q2 < nQ) unions(i2);
     }
-  } else {
 #pragma unroll 1
-    for (uint32_t q = t; q < nQ; q += NT) {
-      const bool colq = q >= nA;
-      const int ps = queue[colq ? G::TV - 1 - (q - nA) : q];
+    while (over) {
+      const int k = __builtin_ctz(over);
+      over &= over - 1u;
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      unions(prep_at(G::pos(vx, vy, vz), ((runs >> k) & 1u) != 0u));
+    }
+  } else {
+    auto one = [&](int ps, bool colq) {
       const int vx = ps & (G::TX - 1), vy = (ps / G::TX) & (G::TY - 1), vz = ps / (G::TX * G::TY);
       const int bx = vx + 1, by = vy + 1, bz = IS3D ? vz + 1 : 0;
       const int b = G::box(bx, by, bz);
@@ -876,6 +895,19 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       const int nl = link_voxel<T, MODE, NB, QK, G, false>(sv, par, b, pn, bx, by, bz, cur, bd, ut, 0u, colq, delta,
                                                             gmin, gmax, y0 + vy == 0);
       if (QK != Q_BLOCK8 && nl == 0) my_prov++;  // block labels are counted separately
+    };
+#pragma unroll 1
+    for (uint32_t q = t; q < nQ; q += NT) {
+      const uint32_t e = queue[q];
+      one((int)(e & 0x7FFFu), (e >> 15) != 0u);
+    }
+#pragma unroll 1
+    while (over) {
+      const int k = __builtin_ctz(over);
+      over &= over - 1u;
+      int vx, vy, vz;
+      G::voxel(t, k, vx, vy, vz);
+      one(G::pos(vx, vy, vz), ((runs >> k) & 1u) != 0u);
     }
   }
   __syncthreads();
@@ -937,16 +969,16 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   for (int k = 0; k < PT; k++) {
     const uint32_t r = root[k];
     // most columns of a component share its root: skip the atomic once set
-    if (r != NONE && (k == 0 || r != root[k - 1]) && !((lload(&L.rbits[r >> 5]) >> (r & 31)) & 1u))
-      atomicOr(&L.rbits[r >> 5], 1u << (r & 31));
+    if (r != NONE && (k == 0 || r != root[k - 1]) && !((lload(&L.w.rbits[r >> 5]) >> (r & 31)) & 1u))
+      atomicOr(&L.w.rbits[r >> 5], 1u << (r & 31));
   }
   __syncthreads();
 
   // 6. rank the marked roots in node (= key) order: one wave scans the bitmask
   if (wv == 0) {
     const int w0 = 2 * lane;
-    const uint32_t c0 = w0 < RW ? (uint32_t)__popc(L.rbits[w0]) : 0u;
-    const uint32_t c1 = w0 + 1 < RW ? (uint32_t)__popc(L.rbits[w0 + 1]) : 0u;
+    const uint32_t c0 = w0 < RW ? (uint32_t)__popc(L.w.rbits[w0]) : 0u;
+    const uint32_t c1 = w0 + 1 < RW ? (uint32_t)__popc(L.w.rbits[w0 + 1]) : 0u;
     uint32_t incl = c0 + c1;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -961,7 +993,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   __syncthreads();
   const uint32_t R = L.nroots;
   auto rank = [&](uint32_t r) -> uint32_t {
-    return L.late.rprefix[r >> 5] + (uint32_t)__popc(L.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
+    return L.late.rprefix[r >> 5] + (uint32_t)__popc(L.w.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
   };
   uint32_t* RLt = p.RL + slot * G::TV;
   // Keys are below 2^32, so they can be formed in 32-bit arithmetic: the
@@ -969,7 +1001,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   // is voxel (x0 + bx - 1, y0 + by - 1, z0 + bz - 1).
   const uint32_t kbox0 = (uint32_t)(x0 - 1) + (uint32_t)sx * ((uint32_t)(y0 - 1) + (uint32_t)sy * (uint32_t)(z0 - 1));
   for (int w = t; w < RW; w += NT) {
-    uint32_t bits = L.rbits[w];
+    uint32_t bits = L.w.rbits[w];
     uint32_t rk = L.late.rprefix[w];
     while (bits) {
       const int bpos = __builtin_ctz(bits);
@@ -1017,7 +1049,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 #pragma unroll
   for (int i = 0; i < HV; i++) {
     const uint32_t r = hroot[i];
-    if (r != NONE && ((L.rbits[r >> 5] >> (r & 31)) & 1u)) rq |= 1u << i;
+    if (r != NONE && ((L.w.rbits[r >> 5] >> (r & 31)) & 1u)) rq |= 1u << i;
   }
   if (__any(rq != 0u)) {
     const uint32_t rc = (uint32_t)__popc(rq);
