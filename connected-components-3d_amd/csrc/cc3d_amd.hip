@@ -8,9 +8,9 @@
 //   [k_periodic] wrap-around unions (cc3d/cc3d.hpp:1048-1073, :1265-1277, :1377-1418)
 //   k_roots      per local root: find + compress; global roots set a bit at their key
 //   k_scan_*     exclusive popcount prefix over the bitmask -> N
-//   k_labels     per local root: final label = 1 + rank of its global root's key
 //   --- stage 2 ---
-//   k_finalize   per voxel: label of its local root, written in the output dtype
+//   k_finalize   per local root: final label = 1 + rank of its global root's key;
+//                per voxel: label of its local root, written in the output dtype
 //   [k_q1_*]     binary 2D-8 only: provisional count and stale background labels
 //
 // Numbering: a component's label is 1 + the number of components whose
@@ -200,8 +200,6 @@ __device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int
 // tens of requests or local roots, so a whole wave per tile would leave most
 // lanes idle.  The label pass does less work per root and runs best wider.
 constexpr int TILE_LANES = 8;
-constexpr int LABEL_LANES = 16;
-constexpr int GLB = 4;  // roots per lane whose label loads are batched in k_labels
 
 // Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests.
 template <class G, int QK>
@@ -459,44 +457,23 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
 
 // Final label of every local root, written over its RL entry: 1 + the rank of
 // its global root's key among the roots of its volume.
-template <class G>
-__global__ __launch_bounds__(WG) void k_labels(GlobalParams p) {
-  const int sub = threadIdx.x % LABEL_LANES;
-  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / LABEL_LANES;
-  const int64_t ngroups = ((int64_t)gridDim.x * WG) / LABEL_LANES;
-  for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
-    const uint32_t R = p.TR[slot];
-    if (R == 0) continue;
-    const uint32_t base = p.nvol ? p.prefix[vol_geo(p, slot).key0 >> 5] : 0u;
-    // GLB roots per lane at a time: each level's loads are issued together
-    for (uint32_t j0 = sub; j0 < R; j0 += LABEL_LANES * GLB) {
-      uint32_t g[GLB], r[GLB], bw[GLB], pf[GLB];
-#pragma unroll
-      for (int u = 0; u < GLB; u++) {
-        const uint32_t j = j0 + u * LABEL_LANES;
-        g[u] = p.RL[slot * G::TV + (j < R ? j : j0)];
-      }
-#pragma unroll
-      for (int u = 0; u < GLB; u++) r[u] = p.GP[g[u]];
-#pragma unroll
-      for (int u = 0; u < GLB; u++) {
-        bw[u] = p.bits[r[u] >> 5];
-        pf[u] = p.prefix[r[u] >> 5];
-      }
-#pragma unroll
-      for (int u = 0; u < GLB; u++) {
-        const uint32_t j = j0 + u * LABEL_LANES;
-        if (j < R) p.RL[slot * G::TV + j] = pf[u] - base + __popc(bw[u] & ((1u << (r[u] & 31)) - 1u)) + 1u;
-      }
-    }
-  }
-}
 
 // Stage 2: every voxel of the tile gets its local root's label.  Thread t
 // owns the column the local kernel gave it: its 8 root indices are one 8- or
 // 16-byte load, issued together with a coalesced copy of the tile's label
 // table into LDS, so the per-voxel lookups hit LDS instead of scattered
 // global reads.
+// Final label of a local root with key g: 1 + the rank of its global root
+// among the global roots (k_roots left GP[g] at the root and marked roots in
+// the bitmask; prefix is the bitmask's exclusive popcount prefix per word).
+// base: the prefix at the volume's first key word (a batch numbers each
+// volume from 1).
+__device__ inline uint32_t root_label(const GlobalParams& p, uint32_t g, uint32_t base) {
+  const uint32_t r = p.GP[g];
+  const uint32_t wd = r >> 5;
+  return p.prefix[wd] - base + __popc(p.bits[wd] & ((1u << (r & 31)) - 1u)) + 1u;
+}
+
 template <class OUT, class G>
 __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_t tile, const VolGeo& v,
                                      OUT* __restrict__ out, uint32_t* lds_labels) {
@@ -520,7 +497,8 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
     for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
   lds_labels[0] = 0u;  // index 0: background
-  for (uint32_t i = t; i < R; i += WG) lds_labels[i + 1] = LT[i];
+  const uint32_t base = p.nvol ? p.prefix[v.key0 >> 5] : 0u;
+  for (uint32_t i = t; i < R; i += WG) lds_labels[i + 1] = root_label(p, LT[i], base);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -562,7 +540,7 @@ __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict
     }
     const uint32_t* __restrict__ LT = p.RL + tile * G::TV;
     lab[n][0] = 0u;
-    for (uint32_t i = t; i < R; i += WG) lab[n][i + 1] = LT[i];
+    for (uint32_t i = t; i < R; i += WG) lab[n][i + 1] = root_label(p, LT[i], 0u);
   }
   __syncthreads();
 #pragma unroll
@@ -928,7 +906,7 @@ static int utab_ready(const uint16_t** out) {
   hipLaunchKernelGGL(k_build_utab<8>, dim3(1), dim3(256), 0, s, tab);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipStreamDestroy(s);
+  (void)hipStreamDestroy(s);  // the build status above is what matters
   if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("building the union tables: ") + hipGetErrorString(e));
   g_utab_dev[dev] = tab;
   *out = tab;
@@ -1141,7 +1119,6 @@ int run_global(cc3d_amd_job* j) {
   const GlobalParams& p = j->gp;
   hipStream_t s = j->stream;
   const unsigned wave_blocks = grid_for(j->g.ntiles, WG / TILE_LANES, 65536);
-  const unsigned label_blocks = grid_for(j->g.ntiles, WG / LABEL_LANES, 65536);
   {
   ProfScope ps_("k_requests", s);
   if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
@@ -1191,8 +1168,6 @@ int run_global(cc3d_amd_job* j) {
   hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)j->g.nblk, p.ctr);
   hipLaunchKernelGGL(k_scan_c, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
   }
-  ProfScope ps_l("k_labels", s);
-  hipLaunchKernelGGL((k_labels<G>), dim3(label_blocks), dim3(WG), 0, s, p);
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
 }
@@ -1834,7 +1809,6 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   }
   HIPCHECK(hipGetLastError());
   const unsigned wave_blocks = grid_for(b->ntiles, WG / TILE_LANES, 65536);
-  const unsigned label_blocks = grid_for(b->ntiles, WG / LABEL_LANES, 65536);
   {
     ProfScope ps_("k_requests", s);
     hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
@@ -1850,8 +1824,6 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     hipLaunchKernelGGL(k_scan_c, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
   }
   {
-    ProfScope ps_("k_labels", s);
-    hipLaunchKernelGGL((k_labels<Geo3>), dim3(label_blocks), dim3(WG), 0, s, p);
     hipLaunchKernelGGL(k_batch_N, dim3((b->n + 63) / 64), dim3(64), 0, s, p.vols, b->n, p.prefix, p.nwords, p.ctr,
                        p.ctr + 1);
   }
