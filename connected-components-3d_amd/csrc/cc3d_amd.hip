@@ -6,7 +6,8 @@
 //   k_local      per tile: LDS union-find, local roots, J tables, requests
 //   k_requests   per tile: cross-tile unions on the global key forest GP
 //   [k_periodic] wrap-around unions (cc3d/cc3d.hpp:1048-1073, :1265-1277, :1377-1418)
-//   k_roots      per local root: find + compress; global roots set a bit at their key
+//   k_roots      per local root: find its global root, store it in RL; global roots
+//                set a bit at their key
 //   k_scan_*     exclusive popcount prefix over the bitmask -> N
 //   --- stage 2 ---
 //   k_finalize   per local root: final label = 1 + rank of its global root's key;
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
       const uint32_t g = p.RL[tile * G::TV + j];
       const uint32_t r = gfind(p.GP, g);
       if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
-      else atomicMin(p.GP + g, r);  // never above a value another find stored
+      p.RL[tile * G::TV + j] = r;  // from here on RL holds each local root's global root
     }
   }
 }
@@ -463,13 +464,12 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
 // 16-byte load, issued together with a coalesced copy of the tile's label
 // table into LDS, so the per-voxel lookups hit LDS instead of scattered
 // global reads.
-// Final label of a local root with key g: 1 + the rank of its global root
-// among the global roots (k_roots left GP[g] at the root and marked roots in
-// the bitmask; prefix is the bitmask's exclusive popcount prefix per word).
-// base: the prefix at the volume's first key word (a batch numbers each
-// volume from 1).
-__device__ inline uint32_t root_label(const GlobalParams& p, uint32_t g, uint32_t base) {
-  const uint32_t r = p.GP[g];
+// Final label of a local root whose global root is r (k_roots stored r in
+// RL): 1 + the rank of r among the global roots, which k_roots marked in the
+// bitmask (prefix: the bitmask's exclusive popcount prefix per word).  base:
+// the prefix at the volume's first key word (a batch numbers each volume
+// from 1).
+__device__ inline uint32_t root_label(const GlobalParams& p, uint32_t r, uint32_t base) {
   const uint32_t wd = r >> 5;
   return p.prefix[wd] - base + __popc(p.bits[wd] & ((1u << (r & 31)) - 1u)) + 1u;
 }
