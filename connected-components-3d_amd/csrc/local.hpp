@@ -374,7 +374,7 @@ struct LocalLds {
   alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
   // storage of steps that are never live together, separated by barriers
   union {
-    uint16_t hqueue[G::IS3D ? G::HALO : 1];  // steps 1-2: halo run starts
+    uint16_t hqueue[G::IS3D ? G::HALO : 1];  // steps 1-2: box indices of halo run starts
     struct {
       uint16_t queue[QC];  // steps 3-4: voxel position | run member << 15
       uint32_t rbits[RW];  // steps 5-8: local roots that own tile voxels
@@ -679,7 +679,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 #pragma unroll
       for (int k = 0; k < HLEN; k++)
         if ((hneed >> k) & 1u)
-          hqueue[hbase++] = (uint16_t)(ring ? G::BX * G::BY + k * RING + t : cbx + G::BX * (hby0 + k));
+          hqueue[hbase++] = (uint16_t)(ring ? G::box(cbx, cby, k + 1) : G::box(cbx, hby0 + k, 0));  // box index
     }
   } else {
 #pragma unroll
@@ -705,7 +705,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   if constexpr (HRUNS) {
     h_nh = hcount;
     if ((uint32_t)t < h_nh) {
-      G::halo_coords(hqueue[t], h_bx, h_by, h_bz);
+      G::template nid_box<Q_NONE>(hqueue[t], h_bx, h_by, h_bz);
       h_b = (uint32_t)G::box(h_bx, h_by, h_bz);
       constexpr uint32_t ALLN = (1u << Nbhd<NB>::N) - 1u;
       const uint32_t lk =
@@ -782,7 +782,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 #pragma unroll 1
     for (uint32_t q = t + NT; q < h_nh; q += NT) {
       int bx, by, bz;
-      G::halo_coords(hqueue[q], bx, by, bz);
+      G::template nid_box<Q_NONE>(hqueue[q], bx, by, bz);
       const int b = G::box(bx, by, bz);
       link_voxel<T, MODE, NB, QK, G, true>(sv, par, b, (uint32_t)b, bx, by, bz, sv[b], Border{}, ut0,
                                            halo_elig<G>(bx, by, bz), false, delta, gmin, gmax, false);
