@@ -17,7 +17,9 @@ Prints one JSON line.  Also reports:
                 steps; traffic from the committed rocprofv3 counter summary
   kernels       every kernel's mean duration, from one extra untimed pass with
                 events around each launch
-  path_roofline the whole call: 8 bytes/voxel (4 in + 4 out) / step time
+  path_roofline the whole call: 8 bytes/voxel (4 in + 4 out) / step time,
+                beside the published peak and the bandwidth a plain device
+                copy of the same volume achieves on this GPU (copy_gbs)
   cpu_baseline  the reference implementation (oracle/_ref, compiled from the
                 reference sources) timed on one host core on the same volume,
                 whose output is also compared bit-for-bit with the GPU's
@@ -59,6 +61,24 @@ def _profile_rows(lib):
     nm = names.raw[i * 64:(i + 1) * 64].split(b"\0", 1)[0].decode()
     rows[nm] = (ms[i], cnt[i])
   return rows
+
+
+def _copy_gbs(src) -> float:
+  """Achievable HBM bandwidth: a device-to-device copy of the tensor src
+  (read + write bytes / time), the best of 10 timed copies."""
+  import torch
+  dst = torch.empty_like(src)
+  dst.copy_(src)
+  best = float("inf")
+  for _ in range(10):
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    dst.copy_(src)
+    b.record()
+    b.synchronize()
+    best = min(best, a.elapsed_time(b) * 1e-3)
+  del dst
+  return 2.0 * src.numel() * src.element_size() / best / 1e9
 
 
 def _traffic_from_profiles(kernel: str):
@@ -410,10 +430,12 @@ def main() -> int:
   }
   path_gbs = 8.0 * V / (ms_per_step * 1e-3) / 1e9
   kernel_sum_ms = sum(v["ms_per_launch"] for v in per_kernel.values())
+  copy_gbs = _copy_gbs(flat)
   path_roofline = {
     "achieved": path_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": path_gbs / HBM_PEAK_GBS,
     "algorithmic_bytes_per_voxel": 8, "kernel_ms_sum": kernel_sum_ms,
     "kernel_frac": (8.0 * V / (kernel_sum_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if kernel_sum_ms else None,
+    "copy_gbs": copy_gbs, "frac_of_copy": path_gbs / copy_gbs if copy_gbs else None,
   }
 
   cpu = None
