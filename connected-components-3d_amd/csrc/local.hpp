@@ -638,6 +638,13 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     uint32_t hneed = 0;  // bit k: halo voxel k of this column is a run start
     if (hcol) {
       const int cpos = ring ? 0 : 1;  // E or K
+      // Does the link to the predecessor cover every eligible position?  For
+      // linkable voxels the eligible set depends on the column only (ring,
+      // z >= 2), or on whether y is the last tile row (bottom plane).
+      const uint32_t cmask = Nbhd<NB>::adj(cpos) | (1u << cpos);
+      auto covers = [&](uint32_t el) { return (cmask & el) == (el & ((1u << Nbhd<NB>::N) - 1u)); };
+      const bool cov_a = covers(ring ? halo_elig<G>(cbx, cby, 2) : halo_elig<G>(cbx, 2, 0));
+      const bool cov_b = covers(halo_elig<G>(cbx, G::TY, 0));
       // a bottom segment after the first continues the run of the voxel
       // before it (staged by the previous segment's thread)
       T prev = (NBS > 1 && !ring) ? hpv : T(0);
@@ -652,11 +659,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         // predecessor in the same neighbouring tile: along z for the ring,
         // rows 1..TY of the bottom plane (the tile below) along y
         const bool linkable = ring ? k >= 1 : (by >= 2 && by <= G::TY);
-        bool member = false;
-        if (fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax)) {
-          const uint32_t el = halo_elig<G>(bx, by, bz);
-          member = ((Nbhd<NB>::adj(cpos) | (1u << cpos)) & el) == (el & ((1u << Nbhd<NB>::N) - 1u));
-        }
+        const bool member = fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax) &&
+                            (ring || by < G::TY ? cov_a : cov_b);
         if (!member) start = (uint32_t)b;
         par[b] = start;
         if (fg(v) && !member) hneed |= 1u << k;
