@@ -346,30 +346,42 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
   else return col_covers_all;
 }
 
-// Waves per SIMD the local kernel is compiled for, which caps its registers:
-// what the LDS of one tile admits (20.4 KB for 4-byte voxels: 8 workgroups
-// of 4 waves per CU, the limit of 8 waves per SIMD, as for 1- and 2-byte
-// voxels; 32 KB for 8-byte voxels: 5).
-template <class T, class G>
-__host__ __device__ constexpr int local_waves() {
-  return sizeof(T) == 8 ? 5 : 8;
+// Work-queue capacity.  A whole tile where LDS allows it: for 1- and
+// 2-byte voxels, and under 6/4-connectivity (FULLQ), where every foreground
+// voxel is queued.  Otherwise the queue gets 42 % of a tile, which keeps 8
+// workgroups per CU for 4-byte voxels and 5 for 8-byte ones: the tiles of
+// the 512^3 uint32 Voronoi workload queue 621 voxels on average and at most
+// 847 of 2048.  Voxels past the capacity are unioned by their own thread
+// after the queue.
+template <class T, class G, bool FULLQ>
+__host__ __device__ constexpr int queue_cap() {
+  return (sizeof(T) <= 2 || FULLQ) ? G::TV : (G::TV * 107 / 256) & ~7;
 }
 
-// Work-queue capacity: 42 % of a tile.  The tiles of the 512^3 Voronoi
-// workload queue 621 voxels on average and at most 847 of 2048; voxels past
-// the capacity are unioned by their own thread after the queue.
-template <class G>
-__host__ __device__ constexpr int queue_cap() {
-  return (G::TV * 107 / 256) & ~7;
+// Waves per SIMD the local kernel is compiled for, which caps its registers:
+// what the LDS of one tile admits.  3D tiles take 13.7 KB (1-byte voxels),
+// 16.7 KB (2), 20.4 KB (4; 22.1 KB with a whole-tile queue) and 32.7 KB (8;
+// 35.1 KB): 8 waves per SIMD (the limit) for 1- and 2-byte voxels and for
+// 4-byte ones with the short queue, 7 for 4-byte voxels with a whole-tile
+// queue, 5 and 4 for 8-byte voxels.
+template <class T, class G, bool FULLQ>
+__host__ __device__ constexpr int local_waves() {
+  return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? (FULLQ ? 7 : 8) : (FULLQ ? 4 : 5);
+}
+
+// Whole-tile queue for a connectivity (6/4: every foreground voxel queued).
+template <int NB>
+__host__ __device__ constexpr bool full_queue() {
+  return NB == 6 || NB == 4;
 }
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
 // several connectivities at run time allocate it once).
-template <class T, class G>
+template <class T, class G, bool FULLQ>
 struct LocalLds {
   static constexpr int NW = G::NT / 64;
   static constexpr int RW = (G::NIDS_MAX + 31) / 32;  // words of the root bitmask
-  static constexpr int QC = queue_cap<G>();
+  static constexpr int QC = queue_cap<T, G, FULLQ>();
   T sv[G::BOXN];
   alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
   // storage of steps that are never live together, separated by barriers
@@ -400,9 +412,9 @@ struct LocalLds {
 // halo pieces join it to the neighbouring tiles' forests.  The first voxel of
 // a column links to its predecessor (the halo voxel below in 3D, the voxel
 // above in 2D) with a plain store, like the rest of the column.
-template <class T, int MODE, int NB, int QK, class G>
+template <class T, int MODE, int NB, int QK, class G, bool FULLQ>
 __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t slot, uint32_t keybase,
-                                  LocalLds<T, G>& L) {
+                                  LocalLds<T, G, FULLQ>& L) {
   constexpr int NT = G::NT;  // threads of the workgroup, one column each
   constexpr int NW = NT / 64;
   constexpr int PT = G::PER_THREAD;
@@ -410,7 +422,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr bool PIECES = !(QK == Q_BIN4 || QK == Q_CONT8);
   constexpr bool IS3D = G::IS3D;
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
-  constexpr int RW = LocalLds<T, G>::RW;
+  constexpr int RW = LocalLds<T, G, FULLQ>::RW;
   static_assert(HPT <= 32 && (!G::IS3D || (G::TZ <= 32 && G::BY <= 32)), "halo piece flags fit one word");
   static_assert(RW <= 128, "one wave scans the root bitmask");
   // 3D transitive 26/18: halo staged as runs (see step 1)
@@ -423,7 +435,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr bool UT = MODE != M_CONT && (NB == 26 || NB == 18 || NB == 8) && QK != Q_CONT8;
   const uint16_t* ut0 = UT ? p.utab + UTab<UT ? NB : 26>::OFF : nullptr;
   uint16_t* queue = L.w.queue;
-  constexpr uint32_t QC = LocalLds<T, G>::QC;
+  constexpr uint32_t QC = LocalLds<T, G, FULLQ>::QC;
   uint16_t* hqueue = L.hqueue;
   uint32_t& qcount = L.qcount;
   uint32_t& nreq = L.nreq;
@@ -1114,8 +1126,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 }
 
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(G::NT, (local_waves<T, G>())) void k_local(LocalParams p) {
-  __shared__ LocalLds<T, G> L;
+__global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<NB>()>())) void k_local(LocalParams p) {
+  __shared__ LocalLds<T, G, full_queue<NB>()> L;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
 }
@@ -1123,12 +1135,13 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G>())) void k_local(LocalPar
 // A batch of 3D volumes in one launch: grid = all tiles of all volumes; each
 // workgroup finds its volume and runs that volume's connectivity.  p carries
 // the shared per-tile arrays; its geometry fields are taken from the table.
-// Holding three connectivities' code, it is compiled for one wave per SIMD
-// fewer than k_local, which keeps its registers from spilling.
+// It takes the whole-tile queue (6-connected volumes share it) and at most 7
+// waves per SIMD: holding three connectivities' code, it needs up to 72
+// VGPRs.
 template <class T, int MODE, class G>
-__global__ __launch_bounds__(G::NT, (local_waves<T, G>() - 1)) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
+__global__ __launch_bounds__(G::NT, (local_waves<T, G, true>() < 7 ? local_waves<T, G, true>() : 7)) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
                                                     int nvol) {
-  __shared__ LocalLds<T, G> L;
+  __shared__ LocalLds<T, G, true> L;
   const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
   const BatchVol& v = vols[find_vol(vtile0, nvol, slot)];
   LocalParams q = p;
