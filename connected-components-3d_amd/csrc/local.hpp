@@ -347,8 +347,8 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
 }
 
 // Work-queue capacity.  A whole tile where LDS allows it: for 1- and
-// 2-byte voxels, and under 6/4-connectivity (FULLQ), where every foreground
-// voxel is queued.  Otherwise the queue gets 42 % of a tile, which keeps 8
+// 2-byte voxels, and under 6/4/18-connectivity (FULLQ), where most
+// foreground voxels are queued.  Otherwise the queue gets 42 % of a tile, which keeps 8
 // workgroups per CU for 4-byte voxels and 5 for 8-byte ones: the tiles of
 // the 512^3 uint32 Voronoi workload queue 621 voxels on average and at most
 // 847 of 2048.  Voxels past the capacity are unioned by their own thread
@@ -369,10 +369,12 @@ __host__ __device__ constexpr int local_waves() {
   return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? (FULLQ ? 7 : 8) : (FULLQ ? 4 : 5);
 }
 
-// Whole-tile queue for a connectivity (6/4: every foreground voxel queued).
+// Whole-tile queue for a connectivity: under 6/4 every foreground voxel is
+// queued, under 18 every foreground run member (a column link leaves two
+// in-plane corners uncovered).
 template <int NB>
 __host__ __device__ constexpr bool full_queue() {
-  return NB == 6 || NB == 4;
+  return NB == 6 || NB == 4 || NB == 18;
 }
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
