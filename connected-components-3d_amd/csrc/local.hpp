@@ -347,8 +347,8 @@ __device__ inline bool column_done(T cur, T prev, bool col_covers_all, T delta, 
 }
 
 // Work-queue capacity.  A whole tile where LDS allows it: for 1- and
-// 2-byte voxels, and under 6/4/18-connectivity (FULLQ), where most
-// foreground voxels are queued.  Otherwise the queue gets 42 % of a tile, which keeps 8
+// 2-byte voxels, and where most foreground voxels are queued (FULLQ:
+// 6/4/18-connectivity, continuous links).  Otherwise the queue gets 42 % of a tile, which keeps 8
 // workgroups per CU for 4-byte voxels and 5 for 8-byte ones: the tiles of
 // the 512^3 uint32 Voronoi workload queue 621 voxels on average and at most
 // 847 of 2048.  Voxels past the capacity are unioned by their own thread
@@ -369,12 +369,13 @@ __host__ __device__ constexpr int local_waves() {
   return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? (FULLQ ? 7 : 8) : (FULLQ ? 4 : 5);
 }
 
-// Whole-tile queue for a connectivity: under 6/4 every foreground voxel is
-// queued, under 18 every foreground run member (a column link leaves two
-// in-plane corners uncovered).
-template <int NB>
+// Whole-tile queue where most foreground voxels are queued: under 6/4 every
+// foreground voxel, under 18 every foreground run member (a column link
+// leaves two in-plane corners uncovered), and for continuous links every
+// run member whose value is not exactly equal to its predecessor's.
+template <int MODE, int NB>
 __host__ __device__ constexpr bool full_queue() {
-  return NB == 6 || NB == 4 || NB == 18;
+  return NB == 6 || NB == 4 || NB == 18 || MODE == M_CONT;
 }
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to
@@ -1128,8 +1129,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 }
 
 template <class T, int MODE, int NB, int QK, class G>
-__global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<NB>()>())) void k_local(LocalParams p) {
-  __shared__ LocalLds<T, G, full_queue<NB>()> L;
+__global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>())) void k_local(LocalParams p) {
+  __shared__ LocalLds<T, G, full_queue<MODE, NB>()> L;
   const int64_t tile = xcd_tile(blockIdx.x, p.ntiles);
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
 }
