@@ -35,20 +35,21 @@ def _fcheck(rc):
     raise RuntimeError(msg)
 
 
-def _device_stats(buf, code, s0, s1, s2, ndim, f_order, N):
-  """counts (uint32), bbox (uint32 min/max pairs), centroids (float64) as device tensors."""
+def _device_stats(buf, code, s0, s1, s2, ndim, f_order, N, bbox=True, centroids=True):
+  """counts (uint32), bbox (uint32 min/max pairs), centroids (float64) as device
+  tensors; bbox / centroids are None when not asked for (not computed)."""
   torch = _torch()
   dev = buf.device
   nd = 2 if ndim == 2 else 3
   counts = torch.empty(N + 1, dtype=torch.int32, device=dev)
-  bbox = torch.empty((N + 1) * 2 * nd, dtype=torch.int32, device=dev)
-  cen = torch.empty((N + 1) * nd, dtype=torch.float64, device=dev)
-  sums = torch.empty((N + 1) * nd, dtype=torch.int64, device=dev)
+  bb = torch.empty((N + 1) * 2 * nd, dtype=torch.int32, device=dev) if bbox else None
+  cen = torch.empty((N + 1) * nd, dtype=torch.float64, device=dev) if centroids else None
+  sums = torch.empty((N + 1) * nd, dtype=torch.int64, device=dev) if centroids else None
+  ptr = lambda t: ctypes.c_void_p(None if t is None else t.data_ptr())
   _fcheck(_bind().cc3d_amd_statistics(
     ctypes.c_void_p(buf.data_ptr()), code, s0, s1, s2, nd, int(f_order), N,
-    ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(bbox.data_ptr()),
-    ctypes.c_void_p(cen.data_ptr()), ctypes.c_void_p(sums.data_ptr()), _stream()))
-  return counts, bbox, cen
+    ptr(counts), ptr(bb), ptr(cen), ptr(sums), _stream()))
+  return counts, bb, cen
 
 
 def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
@@ -101,9 +102,7 @@ def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
     f_order = 1
   else:
     f_order = 0
-  counts, bbox, cen = _device_stats(buf, code, sx, sy, s2 if ndim > 2 else 1, ndim, f_order, N)
-  if ndim == 2 and order == "C":
-    pass
+  counts, bbox, cen = _device_stats(buf, code, sx, sy, s2, ndim, f_order, N)
   nd = 2 if ndim == 2 else 3
   counts = counts.cpu().numpy().view(np.uint32)
   bb = bbox.cpu().numpy().view(np.uint32)
@@ -172,7 +171,7 @@ def dust(img, threshold: Union[int, float, tuple, list], connectivity: int = 26,
   ndim = len(shape)
   sz = shape[2] if ndim > 2 else 1
   counts, _, _ = _device_stats(cc_dev, _NP_TO_CODE[cc_dtype], shape[0], shape[1], sz, ndim,
-                               int(order == "F"), N)
+                               int(order == "F"), N, bbox=False, centroids=False)
 
   if isinstance(threshold, (tuple, list)):
     lo, hi, ranged = float(threshold[0]), float(threshold[1]), 1

@@ -21,7 +21,7 @@ Scheduling:
 
 Footprint of a request (an upper bound, reserved at admission and released at
 completion): labelling workspace + uint32 labels + the dusted copy + the
-statistics arrays for up to one label per voxel.  Results belong to the caller
+counts and bounding boxes for up to one label per voxel.  Results belong to the caller
 after completion; the budget should leave room for the results callers keep.
 """
 from __future__ import annotations
@@ -39,7 +39,7 @@ from ._batch import connected_components_batch
 from ._core import _NP_TO_CODE, _torch, connected_components
 from ._follow import _bind, _device_stats, _fcheck
 
-_STAT_BYTES_PER_LABEL = 4 + 4 * 6 + 8 * 3 + 8 * 3  # counts, bbox, centroids, sums
+_STAT_BYTES_PER_LABEL = 4 + 4 * 6  # counts, bbox
 
 
 class Request:
@@ -83,7 +83,7 @@ def _followons(vol, labels, N: int, dust_threshold: int):
   sx, sy, sz = _dims(vol)
   V = sx * sy * sz
   lab = labels.as_strided((V,), (1,))
-  counts, bbox, _ = _device_stats(lab, L.DT_U32, sx, sy, sz, 3, 1, N)
+  counts, bbox, _ = _device_stats(lab, L.DT_U32, sx, sy, sz, 3, 1, N, centroids=False)
   img = vol.detach().clone(memory_format=torch.preserve_format)
   flat = img.as_strided((V,), (1,))
   erase = torch.empty(N + 1, dtype=torch.uint8, device=vol.device)

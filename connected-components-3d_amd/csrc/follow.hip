@@ -4,9 +4,9 @@
 // statistics: voxel counts (uint32, wrapping like the reference's uint32
 // array), bounding boxes along the numpy axes (min/max per axis) and centroids
 // (sum of coordinates / count in float64; the sums are exact integers, so the
-// result is independent of summation order).  Each thread walks a short
-// contiguous run of voxels and flushes its atomics only when the label
-// changes, so long runs of one label cost one set of atomics.
+// result is independent of summation order).  A workgroup sums a brick of
+// the volume per label in an LDS table and then flushes one set of global
+// atomics per label present in the brick.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,8 +16,6 @@
 #include "common.hpp"
 
 namespace cc3d_amd {
-
-constexpr int STAT_RUN = 16;
 
 template <class L>
 __global__ __launch_bounds__(WG) void k_label_range(const L* lab, int64_t n, int is_signed,
@@ -64,126 +62,201 @@ __global__ void k_stats_init(uint32_t* counts, uint32_t* bbox, unsigned long lon
   for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n1; i += (int64_t)gridDim.x * WG) {
     counts[i] = 0;
     for (int a = 0; a < nd; a++) {
-      bbox[2 * nd * i + 2 * a] = 0xFFFFFFFFu;
-      bbox[2 * nd * i + 2 * a + 1] = 0u;
-      sums[nd * i + a] = 0ull;
+      if (bbox) {
+        bbox[2 * nd * i + 2 * a] = 0xFFFFFFFFu;
+        bbox[2 * nd * i + 2 * a + 1] = 0u;
+      }
+      if (sums) sums[nd * i + a] = 0ull;
     }
   }
 }
 
-// Coordinates along numpy axes (s0, s1, s2) of linear index i: F order has
-// axis 0 fastest, C order axis 2 (or 1 in 2D) fastest.
+// Bricks of the label volume in memory order: m0 is the memory-fastest
+// dimension; p[k] is the numpy axis of memory dimension k (>= nd: a unit axis
+// that is not reported).  A brick is (16 << sl) x (1 << yl) x (1 << zl)
+// voxels, sl + yl + zl = 8: each of the 256 threads walks one 16-voxel
+// segment of one row.
+struct StatGeo {
+  int64_t m0, m1, m2;
+  int64_t nbx, nby, nbricks;
+  int p[3];
+  int nd, sl, yl, zl;
+};
+
+constexpr int ST_SEG = 16;
+constexpr int ST_SLOTS = 512;  // per-brick table; slot 0 is label 0's
+constexpr int ST_PROBES = 16;  // then the run goes straight to global atomics
+
 template <class L>
-__global__ __launch_bounds__(WG) void k_stats(const L* lab, int64_t s0, int64_t s1, int64_t s2, int nd,
-                                              int f_order, uint32_t* counts, uint32_t* bbox,
+__device__ inline void load_seg(const L* p, L (&v)[ST_SEG]) {
+  if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    constexpr int NV = ST_SEG * sizeof(L) / 16;
+    uint4 w[NV];
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int j = 0; j < NV; j++) w[j] = q[j];
+    __builtin_memcpy(v, w, sizeof(v));
+  } else {
+#pragma unroll
+    for (int i = 0; i < ST_SEG; i++) v[i] = p[i];
+  }
+}
+
+// Per-label statistics, W: bit 0 bounding boxes, bit 1 coordinate sums.
+// Each thread splits its segment into runs of one label; each run is added
+// to the brick's LDS table (count, min/max and coordinate sums relative to
+// the brick origin; along a row only the fastest coordinate varies), and
+// after the brick each occupied slot costs one set of global atomics.
+// Label 0, present in most bricks of most volumes, is summed in registers
+// over every brick of the workgroup and flushed once.  A run whose label
+// finds no slot within ST_PROBES probes is flushed to global memory
+// directly.  Counts wrap modulo 2^32 like the reference's uint32 array.
+template <class L, int W>
+__global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t* counts, uint32_t* bbox,
                                               unsigned long long* sums) {
-  const int64_t n = s0 * s1 * s2;
-  const int64_t start = ((int64_t)blockIdx.x * WG + threadIdx.x) * STAT_RUN;
-  auto coords = [&](int64_t i, int64_t& c0, int64_t& c1, int64_t& c2) {
-    if (f_order) {
-      c0 = i % s0;
-      const int64_t r = i / s0;
-      c1 = r % s1;
-      c2 = r / s1;
-    } else {
-      c2 = i % s2;
-      const int64_t r = i / s2;
-      c1 = r % s1;
-      c0 = r / s1;
+  constexpr bool BB = (W & 1) != 0, SM = (W & 2) != 0;
+  constexpr unsigned long long EMPTY = ~0ull;
+  __shared__ unsigned long long tkey[ST_SLOTS];
+  __shared__ uint32_t tcnt[ST_SLOTS];
+  __shared__ uint32_t tmn[BB ? 3 * ST_SLOTS : 1], tmx[BB ? 3 * ST_SLOTS : 1];
+  __shared__ uint32_t tsm[SM ? 3 * ST_SLOTS : 1];
+  const int t = threadIdx.x, nd = g.nd;
+  auto reset = [&](int s) {
+    tkey[s] = EMPTY;
+    tcnt[s] = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if constexpr (BB) {
+        tmn[3 * s + k] = 0xFFFFFFFFu;
+        tmx[3 * s + k] = 0u;
+      }
+      if constexpr (SM) tsm[3 * s + k] = 0u;
     }
   };
-  uint64_t cur = start < n ? (uint64_t)lab[start] : 0;
-  uint32_t cnt = 0;
-  int64_t mn[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx[3] = {-1, -1, -1};
-  unsigned long long sm[3] = {0, 0, 0};
-  // Label 0 (background) is in most runs of most volumes: it is accumulated
-  // per thread and reduced per block, so it costs one set of atomics per
-  // block instead of one per run on a single hot address.
-  uint32_t cnt0 = 0;
+  for (int s = t; s < ST_SLOTS; s += WG) reset(s);
+  const int seg = t & ((1 << g.sl) - 1), row = t >> g.sl;
+  const int lx = seg * ST_SEG, ly = row & ((1 << g.yl) - 1), lz = row >> g.yl;
+  uint32_t c0 = 0;
+  bool seen0 = false;
   int64_t mn0[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx0[3] = {-1, -1, -1};
   unsigned long long sm0[3] = {0, 0, 0};
-  auto flush = [&]() {
-    if (cur == 0) {
-      cnt0 += cnt;
-      for (int a = 0; a < 3; a++) {
-        mn0[a] = mn[a] < mn0[a] ? mn[a] : mn0[a];
-        mx0[a] = mx[a] > mx0[a] ? mx[a] : mx0[a];
-        sm0[a] += sm[a];
-      }
-      return;
-    }
-    atomicAdd(counts + cur, cnt);
-    for (int a = 0; a < nd; a++) {
-      atomicMin(bbox + 2 * nd * cur + 2 * a, (uint32_t)mn[a]);
-      atomicMax(bbox + 2 * nd * cur + 2 * a + 1, (uint32_t)mx[a]);
-      atomicAdd(sums + nd * cur + a, sm[a]);
-    }
-  };
-  if (start < n) {
-    const int64_t end = start + STAT_RUN < n ? start + STAT_RUN : n;
-    for (int64_t i = start; i < end; i++) {
-      const uint64_t l = (uint64_t)lab[i];
-      if (l != cur) {
-        flush();
-        cur = l;
-        cnt = 0;
-        for (int a = 0; a < 3; a++) {
-          mn[a] = INT64_MAX;
-          mx[a] = -1;
-          sm[a] = 0;
+  __syncthreads();
+  for (int64_t b = blockIdx.x; b < g.nbricks; b += gridDim.x) {
+    const int64_t bx = b % g.nbx, r = b / g.nbx, by = r % g.nby, bz = r / g.nby;
+    const int64_t o[3] = {bx << (g.sl + 4), by << g.yl, bz << g.zl};
+    const int64_t x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+    if (x < g.m0 && y < g.m1 && z < g.m2) {
+      const L* p = lab + (x + g.m0 * (y + g.m1 * z));
+      const int len = (int)(g.m0 - x < ST_SEG ? g.m0 - x : ST_SEG);
+      // bit i (1 <= i < len): a run starts at i; bit len ends the last run
+      uint32_t m = 1u << len;
+      if (len == ST_SEG) {
+        L v[ST_SEG];
+        load_seg(p, v);
+#pragma unroll
+        for (int i = 1; i < ST_SEG; i++) m |= (uint32_t)(v[i] != v[i - 1]) << i;
+      } else {
+        L prev = p[0];
+        for (int i = 1; i < len; i++) {
+          const L v = p[i];
+          m |= (uint32_t)(v != prev) << i;
+          prev = v;
         }
       }
-      int64_t c[3];
-      coords(i, c[0], c[1], c[2]);
-      cnt++;
-      for (int a = 0; a < nd; a++) {
-        mn[a] = c[a] < mn[a] ? c[a] : mn[a];
-        mx[a] = c[a] > mx[a] ? c[a] : mx[a];
-        sm[a] += (unsigned long long)c[a];
-      }
-    }
-    flush();
-  }
-  // block reduction of the background accumulators
+      for (int xs = 0; xs < len;) {
+        const int xe = __builtin_ctz(m & ~((2u << xs) - 1u));
+        const unsigned long long key = (unsigned long long)p[xs];
+        const uint32_t c = (uint32_t)(xe - xs);
+        const int xa = lx + xs, xb = lx + xe - 1;
+        int s = 0;
+        if (key != 0) {
+          s = -1;
+          uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 55);
+          for (int q = 0; q < ST_PROBES; q++) {
+            h = h ? h : 1u;
+            unsigned long long k = *reinterpret_cast<volatile unsigned long long*>(tkey + h);
+            if (k == EMPTY) k = atomicCAS(tkey + h, EMPTY, key);
+            if (k == EMPTY || k == key) {
+              s = (int)h;
+              break;
+            }
+            h = (h + 1) & (ST_SLOTS - 1);
+          }
+        }
+        const uint32_t lo[3] = {(uint32_t)xa, (uint32_t)ly, (uint32_t)lz};
+        const uint32_t hi[3] = {(uint32_t)xb, (uint32_t)ly, (uint32_t)lz};
+        const uint32_t ls[3] = {(uint32_t)(xa + xb) * c / 2u, c * (uint32_t)ly, c * (uint32_t)lz};
+        if (s >= 0) {
+          atomicAdd(tcnt + s, c);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    cnt0 += __shfl_xor(cnt0, o);
-    for (int a = 0; a < 3; a++) {
-      const int64_t m1 = __shfl_xor(mn0[a], o), m2 = __shfl_xor(mx0[a], o);
-      mn0[a] = m1 < mn0[a] ? m1 : mn0[a];
-      mx0[a] = m2 > mx0[a] ? m2 : mx0[a];
-      sm0[a] += __shfl_xor(sm0[a], o);
-    }
-  }
-  __shared__ uint32_t bc[WG / 64];
-  __shared__ long long bmn[WG / 64][3], bmx[WG / 64][3];
-  __shared__ unsigned long long bsm[WG / 64][3];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) {
-    bc[wv] = cnt0;
-    for (int a = 0; a < 3; a++) {
-      bmn[wv][a] = mn0[a];
-      bmx[wv][a] = mx0[a];
-      bsm[wv][a] = sm0[a];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < WG / 64; w++) {
-      cnt0 += bc[w];
-      for (int a = 0; a < 3; a++) {
-        mn0[a] = bmn[w][a] < mn0[a] ? bmn[w][a] : mn0[a];
-        mx0[a] = bmx[w][a] > mx0[a] ? bmx[w][a] : mx0[a];
-        sm0[a] += bsm[w][a];
+          for (int k = 0; k < 3; k++) {
+            if constexpr (BB) {
+              atomicMin(tmn + 3 * s + k, lo[k]);
+              atomicMax(tmx + 3 * s + k, hi[k]);
+            }
+            if constexpr (SM) atomicAdd(tsm + 3 * s + k, ls[k]);
+          }
+        } else {
+          atomicAdd(counts + key, c);
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            const int a = g.p[k];
+            if (a >= nd) continue;
+            if constexpr (BB) {
+              atomicMin(bbox + 2 * nd * key + 2 * a, (uint32_t)(o[k] + lo[k]));
+              atomicMax(bbox + 2 * nd * key + 2 * a + 1, (uint32_t)(o[k] + hi[k]));
+            }
+            if constexpr (SM) atomicAdd(sums + nd * key + a, (unsigned long long)c * o[k] + ls[k]);
+          }
+        }
+        xs = xe;
       }
     }
-    if (cnt0) {
-      atomicAdd(counts, cnt0);
-      for (int a = 0; a < nd; a++) {
-        atomicMin(bbox + 2 * a, (uint32_t)mn0[a]);
-        atomicMax(bbox + 2 * a + 1, (uint32_t)mx0[a]);
-        atomicAdd(sums + a, sm0[a]);
+    __syncthreads();
+    for (int s = t; s < ST_SLOTS; s += WG) {
+      const uint32_t c = tcnt[s];
+      if (c == 0) continue;  // never claimed: a claimed slot always has a run added
+      if (s == 0) {
+        c0 += c;
+        seen0 = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if constexpr (BB) {
+            const int64_t a = o[k] + tmn[k], z2 = o[k] + tmx[k];
+            mn0[k] = a < mn0[k] ? a : mn0[k];
+            mx0[k] = z2 > mx0[k] ? z2 : mx0[k];
+          }
+          if constexpr (SM) sm0[k] += (unsigned long long)c * o[k] + tsm[k];
+        }
+      } else {
+        const unsigned long long key = tkey[s];
+        atomicAdd(counts + key, c);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const int a = g.p[k];
+          if (a >= nd) continue;
+          if constexpr (BB) {
+            atomicMin(bbox + 2 * nd * key + 2 * a, (uint32_t)(o[k] + tmn[3 * s + k]));
+            atomicMax(bbox + 2 * nd * key + 2 * a + 1, (uint32_t)(o[k] + tmx[3 * s + k]));
+          }
+          if constexpr (SM) atomicAdd(sums + nd * key + a, (unsigned long long)c * o[k] + tsm[3 * s + k]);
+        }
       }
+      reset(s);
+    }
+    __syncthreads();
+  }
+  if (t == 0 && seen0) {
+    atomicAdd(counts, c0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int a = g.p[k];
+      if (a >= nd) continue;
+      if constexpr (BB) {
+        atomicMin(bbox + 2 * a, (uint32_t)mn0[k]);
+        atomicMax(bbox + 2 * a + 1, (uint32_t)mx0[k]);
+      }
+      if constexpr (SM) atomicAdd(sums + a, sm0[k]);
     }
   }
 }
@@ -232,6 +305,32 @@ static inline unsigned fgrid(int64_t n, int64_t per, int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + per - 1) / per, cap));
 }
 
+// One workgroup per brick, at most as many as are resident at once: each
+// workgroup then walks its bricks in a grid-stride loop.
+template <class L, int W>
+static void launch_stats_w(const void* labels, const StatGeo& g, uint32_t* counts, uint32_t* bbox,
+                           unsigned long long* sums, hipStream_t s) {
+  int dev = 0, cus = 0, occ = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_stats<L, W>), WG, 0);
+  const int64_t cap = (int64_t)std::max(cus, 1) * std::max(occ, 1);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(g.nbricks, cap));
+  hipLaunchKernelGGL((k_stats<L, W>), dim3(grid), dim3(WG), 0, s, static_cast<const L*>(labels), g, counts, bbox,
+                     sums);
+}
+
+template <class L>
+static void launch_stats(int w, const void* labels, const StatGeo& g, uint32_t* counts, uint32_t* bbox,
+                         unsigned long long* sums, hipStream_t s) {
+  switch (w) {
+    case 0: launch_stats_w<L, 0>(labels, g, counts, bbox, sums, s); break;
+    case 1: launch_stats_w<L, 1>(labels, g, counts, bbox, sums, s); break;
+    case 2: launch_stats_w<L, 2>(labels, g, counts, bbox, sums, s); break;
+    default: launch_stats_w<L, 3>(labels, g, counts, bbox, sums, s); break;
+  }
+}
+
 }  // namespace cc3d_amd
 
 using namespace cc3d_amd;
@@ -267,24 +366,61 @@ int cc3d_amd_label_range(const void* labels, int dtype, int64_t voxels, int is_s
 }
 
 // counts: uint32[N+1]; bbox: uint32[(N+1)*2*nd] as min0,max0,min1,max1[,min2,max2]
-// (min = 0xFFFFFFFF, max = 0 for absent labels); centroids: float64[(N+1)*nd].
-// sums_scratch: uint64[(N+1)*nd].
+// (min = 0xFFFFFFFF, max = 0 for absent labels); centroids: float64[(N+1)*nd];
+// sums_scratch: uint64[(N+1)*nd].  bbox and centroids may be null (not
+// computed); centroids need sums_scratch.
 int cc3d_amd_statistics(const void* labels, int dtype, int64_t s0, int64_t s1, int64_t s2, int ndim, int f_order,
                         uint64_t N, uint32_t* counts, uint32_t* bbox, double* centroids,
                         unsigned long long* sums_scratch, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!counts || (centroids && !sums_scratch)) return ffail(CC3D_AMD_ERR_INVALID, "statistics needs counts (and sums_scratch for centroids)");
+  if (dtype != DT_U8 && dtype != DT_U16 && dtype != DT_U32 && dtype != DT_U64)
+    return ffail(CC3D_AMD_ERR_INVALID, "labels must be an integer type");
   const int64_t n1 = (int64_t)N + 1, n = s0 * s1 * s2;
   const int nd = ndim == 2 ? 2 : 3;
-  hipLaunchKernelGGL(k_stats_init, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, counts, bbox, sums_scratch, n1, nd);
-  const dim3 g(fgrid(n, (int64_t)WG * STAT_RUN, 1 << 30));
-  switch (dtype) {
-    case DT_U8: hipLaunchKernelGGL((k_stats<uint8_t>), g, dim3(WG), 0, s, (const uint8_t*)labels, s0, s1, s2, nd, f_order, counts, bbox, sums_scratch); break;
-    case DT_U16: hipLaunchKernelGGL((k_stats<uint16_t>), g, dim3(WG), 0, s, (const uint16_t*)labels, s0, s1, s2, nd, f_order, counts, bbox, sums_scratch); break;
-    case DT_U32: hipLaunchKernelGGL((k_stats<uint32_t>), g, dim3(WG), 0, s, (const uint32_t*)labels, s0, s1, s2, nd, f_order, counts, bbox, sums_scratch); break;
-    case DT_U64: hipLaunchKernelGGL((k_stats<uint64_t>), g, dim3(WG), 0, s, (const uint64_t*)labels, s0, s1, s2, nd, f_order, counts, bbox, sums_scratch); break;
-    default: return ffail(CC3D_AMD_ERR_INVALID, "labels must be an integer type");
+  unsigned long long* sums = centroids ? sums_scratch : nullptr;
+  hipLaunchKernelGGL(k_stats_init, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, counts, bbox, sums, n1, nd);
+  if (n > 0) {
+    StatGeo g;
+    int64_t m[3];
+    if (f_order) {
+      m[0] = s0, m[1] = s1, m[2] = s2;
+      g.p[0] = 0, g.p[1] = 1, g.p[2] = 2;
+    } else {
+      m[0] = s2, m[1] = s1, m[2] = s0;
+      g.p[0] = 2, g.p[1] = 1, g.p[2] = 0;
+    }
+    for (int r = 0; r < 2 && m[0] == 1; r++) {  // a unit fastest dimension: the next one is contiguous
+      const int64_t m0 = m[0];
+      const int p0 = g.p[0];
+      m[0] = m[1], m[1] = m[2], m[2] = m0;
+      g.p[0] = g.p[1], g.p[1] = g.p[2], g.p[2] = p0;
+    }
+    g.m0 = m[0], g.m1 = m[1], g.m2 = m[2];
+    g.nd = nd;
+    g.sl = g.m0 >= 48 ? 2 : g.m0 >= 24 ? 1 : 0;
+    while (g.sl < 8 && g.m1 * g.m2 < (1ll << (8 - g.sl)) && (16ll << g.sl) < g.m0) g.sl++;  // few long rows
+    const int rl = 8 - g.sl;
+    if (g.m2 == 1) {
+      g.yl = rl;
+    } else {
+      g.yl = (rl + 1) / 2;
+      while (g.yl > 0 && (1ll << (g.yl - 1)) >= g.m1) g.yl--;
+    }
+    g.zl = rl - g.yl;
+    g.nbx = (g.m0 + (16ll << g.sl) - 1) >> (g.sl + 4);
+    g.nby = (g.m1 + (1ll << g.yl) - 1) >> g.yl;
+    g.nbricks = g.nbx * g.nby * ((g.m2 + (1ll << g.zl) - 1) >> g.zl);
+    const int w = (bbox ? 1 : 0) | (sums ? 2 : 0);
+    switch (dtype) {
+      case DT_U8: launch_stats<uint8_t>(w, labels, g, counts, bbox, sums, s); break;
+      case DT_U16: launch_stats<uint16_t>(w, labels, g, counts, bbox, sums, s); break;
+      case DT_U32: launch_stats<uint32_t>(w, labels, g, counts, bbox, sums, s); break;
+      default: launch_stats<uint64_t>(w, labels, g, counts, bbox, sums, s); break;
+    }
   }
-  hipLaunchKernelGGL(k_centroids, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, counts, sums_scratch, centroids, n1, nd);
+  if (centroids)
+    hipLaunchKernelGGL(k_centroids, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, counts, sums, centroids, n1, nd);
   if (hipGetLastError() != hipSuccess) return ffail(CC3D_AMD_ERR_HIP, "statistics launch failed");
   return CC3D_AMD_OK;
 }

@@ -154,7 +154,8 @@ int cc3d_amd_label_range(const void* labels, int dtype, int64_t voxels, int is_s
  * fastest in memory.  Outputs (device): counts uint32[N+1];
  * bbox uint32[(N+1)*2*ndim] as min,max per axis (min 0xFFFFFFFF, max 0 for
  * absent labels); centroids float64[(N+1)*ndim] (NaN for absent labels);
- * sums_scratch uint64[(N+1)*ndim]. */
+ * sums_scratch uint64[(N+1)*ndim].  bbox and centroids may be NULL: that
+ * output is then not computed (centroids need sums_scratch). */
 int cc3d_amd_statistics(const void* labels, int dtype, int64_t s0, int64_t s1, int64_t s2,
                         int ndim, int f_order, uint64_t N, uint32_t* counts, uint32_t* bbox,
                         double* centroids, unsigned long long* sums_scratch, void* stream);

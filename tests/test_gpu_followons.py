@@ -41,17 +41,56 @@ def _np_stats(labels):
 @pytest.mark.parametrize("order", ["C", "F"])
 @pytest.mark.parametrize("shape", [(40, 33, 21), (70, 45), (200,)])
 def test_statistics_match_numpy(gpu, order, shape):
-  import cc3d
   rng = np.random.default_rng(len(shape))
   lab = rng.integers(0, 30, size=shape).astype(np.uint32)
   lab[lab == 7] = 0  # an absent label
-  lab = np.asfortranarray(lab) if order == "F" else np.ascontiguousarray(lab)
+  _check_stats(np.asfortranarray(lab) if order == "F" else np.ascontiguousarray(lab))
+
+
+def _check_stats(lab):
+  import cc3d
   st = cc3d.statistics(lab, no_slice_conversion=True)
   counts, bb, cen = _np_stats(lab)
   assert np.array_equal(st["voxel_counts"], counts)
   assert np.array_equal(st["bounding_boxes"].astype(np.int64), bb)
   assert np.array_equal(np.isnan(st["centroids"]), np.isnan(cen))
   assert np.array_equal(np.nan_to_num(st["centroids"]), np.nan_to_num(cen))
+
+
+@pytest.mark.parametrize("shape,order,dtype", [
+  ((1, 50, 40), "F", np.uint16),    # unit fastest axis: the next axis is contiguous
+  ((40, 50, 1), "C", np.uint32),
+  ((1, 1, 300), "F", np.uint8),
+  ((130, 17, 9), "F", np.uint64),   # partial bricks on every axis
+  ((9, 17, 130), "C", np.uint32),
+  ((5000, 3), "F", np.uint32),      # few long rows
+  ((3, 70000), "C", np.uint16),
+])
+def test_statistics_brick_edges(gpu, shape, order, dtype):
+  print("statistics brick case", shape, order, np.dtype(dtype).name, flush=True)
+  rng = np.random.default_rng(11)
+  lab = rng.integers(0, 12, size=shape).astype(dtype)
+  lab = np.asfortranarray(lab) if order == "F" else np.ascontiguousarray(lab)
+  _check_stats(lab)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+def test_statistics_table_overflow(gpu, dtype):
+  """Nearly every voxel its own label: most runs find no free slot in the
+  per-brick table and go straight to global memory."""
+  rng = np.random.default_rng(12)
+  lab = rng.integers(0, 64 * 64 * 16, size=(64, 64, 16)).astype(dtype)  # labels < voxels
+  lab[:, :, 3] = 0
+  _check_stats(np.asfortranarray(lab))
+
+
+def test_statistics_many_bricks_per_workgroup(gpu):
+  """A segmented volume with more bricks than resident workgroups, so each
+  workgroup walks several bricks and carries label 0's totals across them."""
+  import synth
+  lab = synth.voronoi((256, 256, 160), 3000, seed=13, id_range=(1, 400000))
+  _check_stats(lab)
+  _check_stats(np.ascontiguousarray(lab[:, :, :97]))
 
 
 def test_statistics_reference_known_answer(gpu):
