@@ -1104,7 +1104,15 @@ struct cc3d_amd_batch {
          oPRE = 0, oBSUM = 0, oCTR = 0;
   BatchVol* h_tab = nullptr;  // pinned host copy of the volume table
   int64_t* h_t0 = nullptr;    // pinned: first tile of each volume (n + 1)
-  Counters* h_ctr = nullptr;  // pinned: entry 0 whole batch, 1 + v volume v
+  Counters* h_ctr = nullptr;  // pinned: entry 0 whole batch, 1 + k table entry k
+  // The table holds the volumes grouped by connectivity (one local launch
+  // per group); perm[k] is the caller's index of table entry k.
+  struct Group {
+    int nb;
+    int64_t tile0, tiles;
+  };
+  std::vector<int> perm;
+  std::vector<Group> groups;
   bool outputs_set = false, capturing = false;
   GlobalParams gp{};
   LocalParams lp{};
@@ -1602,9 +1610,10 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   if (!vols || n <= 0) return fail(CC3D_AMD_ERR_INVALID, "a batch needs at least one volume");
   tab.resize(n);
   t0.resize(n + 1);
-  uint64_t key = 0;
-  int64_t tiles = 0;
-  for (int i = 0; i < n; i++) {
+  b.perm.resize(n);
+  b.groups.clear();
+  std::vector<int> nbs(n);
+  for (int i = 0; i < n; i++) {  // checked in the caller's order
     const cc3d_amd_volume& v = vols[i];
     if (!v.in) return fail(CC3D_AMD_ERR_INVALID, "NULL input in batch");
     if (v.sx <= 0 || v.sy <= 0 || v.sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
@@ -1618,7 +1627,16 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
     } else {
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     }
-    BatchVol& t = tab[i];
+    nbs[i] = nb;
+    b.perm[i] = i;
+  }
+  std::stable_sort(b.perm.begin(), b.perm.end(), [&](int x, int y) { return nbs[x] > nbs[y]; });
+  uint64_t key = 0;
+  int64_t tiles = 0;
+  for (int k = 0; k < n; k++) {
+    const int i = b.perm[k], nb = nbs[i];
+    const cc3d_amd_volume& v = vols[i];
+    BatchVol& t = tab[k];
     t.in = v.in;
     t.out = nullptr;
     t.sx = v.sx;
@@ -1635,7 +1653,9 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
     t.nkeys = (uint32_t)std::min<uint64_t>(V, 0xFFFFFFFFull);
     t.nb = nb;
     t.out_dtype = DT_U32;
-    t0[i] = tiles;
+    t0[k] = tiles;
+    if (b.groups.empty() || b.groups.back().nb != nb) b.groups.push_back({nb, tiles, 0});
+    b.groups.back().tiles += t.ntiles;
     tiles += t.ntiles;
     key += (V + 31) & ~(uint64_t)31;
     if (key >= 0xFFFFFFFFull) return fail(CC3D_AMD_ERR_INVALID, "a batch must hold fewer than 2^32 - 1 voxels");
@@ -1777,9 +1797,9 @@ int cc3d_amd_batch_set_outputs(cc3d_amd_batch* b, void* const* outs, const int* 
   }
   // the pinned table may still feed an earlier upload
   HIPCHECK(hipStreamSynchronize(b->stream));
-  for (int i = 0; i < b->n; i++) {
-    b->h_tab[i].out = outs[i];
-    b->h_tab[i].out_dtype = out_dtypes[i];
+  for (int k = 0; k < b->n; k++) {
+    b->h_tab[k].out = outs[b->perm[k]];
+    b->h_tab[k].out_dtype = out_dtypes[b->perm[k]];
   }
   HIPCHECK(hipMemcpyAsync(b->ws + b->oVOL, b->h_tab, b->n * sizeof(BatchVol), hipMemcpyHostToDevice, b->stream));
   b->outputs_set = true;
@@ -1796,11 +1816,16 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
                        p.ctr, b->n + 1, p.bits, b->nwords);
   }
   HIPCHECK(hipGetLastError());
-  hipError_t e;
+  hipError_t e = hipSuccess;
   {
     ProfScope ps_("k_local", s);
-    if (b->mode == M_BINARY) e = launch_local_batch_binary(b->dt, b->lp, p.vols, p.vtile0, b->n, s);
-    else e = launch_local_batch_multi(b->dt, b->lp, p.vols, p.vtile0, b->n, s);
+    for (const auto& g : b->groups) {
+      if (b->mode == M_BINARY)
+        e = launch_local_batch_binary(b->dt, g.nb, b->lp, p.vols, p.vtile0, b->n, g.tile0, g.tiles, s);
+      else
+        e = launch_local_batch_multi(b->dt, g.nb, b->lp, p.vols, p.vtile0, b->n, g.tile0, g.tiles, s);
+      if (e != hipSuccess) break;
+    }
   }
   if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_local_batch: ") + hipGetErrorString(e));
   {
@@ -1835,13 +1860,14 @@ int cc3d_amd_batch_stats(cc3d_amd_batch* b, cc3d_amd_stats* st) {
   if (!b || !st) return fail(CC3D_AMD_ERR_INVALID, "NULL argument");
   HIPCHECK(hipMemcpyAsync(b->h_ctr, b->ws + b->oCTR, (b->n + 1) * sizeof(Counters), hipMemcpyDeviceToHost, b->stream));
   HIPCHECK(hipStreamSynchronize(b->stream));
-  for (int i = 0; i < b->n; i++) {
-    const Counters& c = b->h_ctr[i + 1];
-    st[i].N = c.N;
-    st[i].provisional = c.prov;
-    st[i].epl = c.epl;
-    st[i].first_foreground_row = c.last_row < 0 ? -1 : c.first_row;
-    st[i].last_foreground_row = c.last_row;
+  for (int k = 0; k < b->n; k++) {
+    const Counters& c = b->h_ctr[k + 1];
+    cc3d_amd_stats& o = st[b->perm[k]];
+    o.N = c.N;
+    o.provisional = c.prov;
+    o.epl = c.epl;
+    o.first_foreground_row = c.last_row < 0 ? -1 : c.first_row;
+    o.last_foreground_row = c.last_row;
   }
   return CC3D_AMD_OK;
 }

@@ -1135,17 +1135,16 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>()
   local_tile<T, MODE, NB, QK, G>(p, tile, tile, 0u, L);
 }
 
-// A batch of 3D volumes in one launch: grid = all tiles of all volumes; each
-// workgroup finds its volume and runs that volume's connectivity.  p carries
-// the shared per-tile arrays; its geometry fields are taken from the table.
-// It takes the whole-tile queue (6-connected volumes share it) and at most 7
-// waves per SIMD: holding three connectivities' code, it needs up to 72
-// VGPRs.
-template <class T, int MODE, class G>
-__global__ __launch_bounds__(G::NT, (local_waves<T, G, true>() < 7 ? local_waves<T, G, true>() : 7)) void k_local_batch(LocalParams p, const BatchVol* vols, const int64_t* vtile0,
-                                                    int nvol) {
-  __shared__ LocalLds<T, G, true> L;
-  const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
+// The volumes of a batch that share a connectivity, in one launch: their
+// tiles are contiguous in the batch, [gtile0, gtile0 + gtiles); each
+// workgroup finds its volume in the table.  p carries the shared per-tile
+// arrays; the geometry fields come from the table.  Queue capacity and
+// occupancy are those of the single-volume kernel of the same connectivity.
+template <class T, int MODE, int NB, class G>
+__global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>())) void k_local_batch(
+  LocalParams p, const BatchVol* vols, const int64_t* vtile0, int nvol, int64_t gtile0, int64_t gtiles) {
+  __shared__ LocalLds<T, G, full_queue<MODE, NB>()> L;
+  const int64_t slot = gtile0 + xcd_tile(blockIdx.x, gtiles);
   const BatchVol& v = vols[find_vol(vtile0, nvol, slot)];
   LocalParams q = p;
   q.in = v.in;
@@ -1157,34 +1156,44 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, true>() < 7 ? local_waves
   q.nty = v.nty;
   q.ntz = v.ntz;
   q.ntiles = v.ntiles;
-  const int64_t tile = slot - v.tile0;
-  if (v.nb == 26) local_tile<T, MODE, 26, Q_NONE, G>(q, tile, slot, v.key0, L);
-  else if (v.nb == 18) local_tile<T, MODE, 18, Q_NONE, G>(q, tile, slot, v.key0, L);
-  else local_tile<T, MODE, 6, Q_NONE, G>(q, tile, slot, v.key0, L);
+  local_tile<T, MODE, NB, Q_NONE, G, full_queue<MODE, NB>()>(q, slot - v.tile0, slot, v.key0, L);
 }
 
 // Launchers (one translation unit per mode keeps compile times short).
 hipError_t launch_local_multi(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
 hipError_t launch_local_binary(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
 hipError_t launch_local_cont(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
-hipError_t launch_local_batch_multi(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
-                                    int nvol, hipStream_t s);
-hipError_t launch_local_batch_binary(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
-                                     int nvol, hipStream_t s);
+// A batch launches once per connectivity: tiles [gtile0, gtile0 + gtiles) all
+// belong to volumes of connectivity nb.
+hipError_t launch_local_batch_multi(int dt, int nb, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
+                                    int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols,
+                                     const int64_t* vtile0, int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s);
 
 template <int MODE>
-hipError_t launch_local_batch_mode(int dt, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
-                                   int nvol, hipStream_t s) {
-  const dim3 grid((unsigned)p.ntiles), block(Geo3::NT);
+hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
+                                   int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s) {
+  const dim3 grid((unsigned)gtiles), block(Geo3::NT);
+#define CC3D_B(T, NB) \
+  hipLaunchKernelGGL((k_local_batch<T, MODE, NB, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol, gtile0, gtiles)
+#define CC3D_BNB(T)                          \
+  switch (nb) {                              \
+    case 26: CC3D_B(T, 26); break;           \
+    case 18: CC3D_B(T, 18); break;           \
+    case 6: CC3D_B(T, 6); break;             \
+    default: return hipErrorInvalidValue;    \
+  }
   switch (dt) {
-    case DT_U8: hipLaunchKernelGGL((k_local_batch<uint8_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
-    case DT_U16: hipLaunchKernelGGL((k_local_batch<uint16_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
-    case DT_U32: hipLaunchKernelGGL((k_local_batch<uint32_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
-    case DT_U64: hipLaunchKernelGGL((k_local_batch<uint64_t, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
-    case DT_F32: hipLaunchKernelGGL((k_local_batch<float, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
-    case DT_F64: hipLaunchKernelGGL((k_local_batch<double, MODE, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol); break;
+    case DT_U8: CC3D_BNB(uint8_t); break;
+    case DT_U16: CC3D_BNB(uint16_t); break;
+    case DT_U32: CC3D_BNB(uint32_t); break;
+    case DT_U64: CC3D_BNB(uint64_t); break;
+    case DT_F32: CC3D_BNB(float); break;
+    case DT_F64: CC3D_BNB(double); break;
     default: return hipErrorInvalidValue;
   }
+#undef CC3D_BNB
+#undef CC3D_B
   return hipGetLastError();
 }
 
