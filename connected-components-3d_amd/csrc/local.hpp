@@ -175,6 +175,29 @@ __device__ inline T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   }
 }
 
+// 6-connected voxel p (box index b): the links to M (position 2) and K (1)
+// and whether each still needs a union here, given whether p is linked to E
+// (0).  A phantom path x - c - y (both pairs unioned in this tile) makes the
+// x - y union redundant; positions E K M B D J = 0 1 2 3 5 7.  Bits: 0 union
+// M, 1 union K, 2 M linked, 3 K linked.
+template <class T, int MODE, class G, bool HALO>
+__device__ inline uint32_t six_links(const T* sv, int b, T cur, const Border& bd, uint32_t elig, bool lE) {
+  auto lk = [&](int i) -> bool {
+    if (HALO && !((elig >> i) & 1u)) return false;
+    const T q = sv[b + box_off<G>(i)];
+    return MODE == M_MULTI ? (q == cur) : fg(q);
+  };
+  auto path = [&](int x, int c, int y) -> bool {
+    if (HALO) return true;
+    return ((proc_mask<true>(x, bd) >> c) & 1u) && ((proc_mask<true>(c, bd) >> y) & 1u);
+  };
+  const bool lM = lk(2);
+  const bool uM = lM && !(lE && lk(5) && path(0, 5, 2));
+  const bool lK = lk(1);
+  const bool uK = lK && !((lE && lk(3) && path(0, 3, 1)) || (lM && lk(7) && path(2, 7, 1)));
+  return (uM ? 1u : 0u) | (uK ? 2u : 0u) | (lM ? 4u : 0u) | (lK ? 8u : 0u);
+}
+
 // Unions for one foreground voxel p (box index b, node pn, box coordinates
 // bx, by, bz).  HALO: a halo voxel, restricted to the positions in `elig`
 // (halo voxels of the same neighbouring tile).  COL: p continues a run, so
@@ -254,14 +277,12 @@ __device__ inline int link_voxel(const T* sv, uint16_t* par, int b, uint32_t pn,
       return ((proc_mask<IS3D>(x, bd) >> c) & 1u) && ((proc_mask<IS3D>(c, bd) >> y) & 1u);
     };
     if constexpr (NB == 6) {
-      // positions E K M B D J = 0 1 2 3 5 7
       const bool lE = col || lk(0);
       if (lE && !col) a = lunion(par, a, nd(0));
-      const bool lM = lk(2);
-      if (lM && !(lE && lk(5) && path(0, 5, 2))) a = lunion(par, a, nd(2));
-      const bool lK = lk(1);
-      if (lK && !((lE && lk(3) && path(0, 3, 1)) || (lM && lk(7) && path(2, 7, 1)))) lunion(par, a, nd(1));
-      return (int)lE + (int)lM + (int)lK;
+      const uint32_t s6 = six_links<T, MODE, G, HALO>(sv, b, cur, bd, elig, lE);
+      if (s6 & 1u) a = lunion(par, a, nd(2));
+      if (s6 & 2u) lunion(par, a, nd(1));
+      return (int)lE + (int)((s6 >> 2) & 1u) + (int)((s6 >> 3) & 1u);
     } else {
       // positions K M J = 0 1 2
       const bool lK = col || lk(0);
@@ -771,10 +792,17 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       const int row = vy + G::TY * vz;
       my_first = row < my_first ? row : my_first;
       my_last = row > my_last ? row : my_last;
-      if (!((runs >> k) & 1u) ||
-          !column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0, delta,
-                                        gmin, gmax))
-        need |= 1u << k;
+      bool q = !((runs >> k) & 1u) ||
+               !column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0,
+                                             delta, gmin, gmax);
+      if constexpr (NB == 6 && MODE != M_CONT) {
+        // a 6-connected run member is queued only for a union it still needs
+        if (q && ((runs >> k) & 1u)) {
+          const Border bd = make_border<true>(vx == 0, vx == G::TX - 1, vy == 0, vy == G::TY - 1, vz == 0);
+          q = (six_links<T, MODE, G, false>(sv, b0 + k * CSTEP, cur, bd, 0u, true) & 3u) != 0u;
+        }
+      }
+      if (q) need |= 1u << k;
     }
     // wave-level exclusive scan of the counts, one LDS atomic per wave
     const uint32_t cnt = (uint32_t)__popc(need);
