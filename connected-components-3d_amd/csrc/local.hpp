@@ -390,13 +390,15 @@ __host__ __device__ constexpr int local_waves() {
   return sizeof(T) <= 2 ? 8 : sizeof(T) == 4 ? (FULLQ ? 7 : 8) : (FULLQ ? 4 : 5);
 }
 
-// Whole-tile queue where most foreground voxels are queued: under 6/4 every
-// foreground voxel, under 18 every foreground run member (a column link
-// leaves two in-plane corners uncovered), and for continuous links every
-// run member whose value is not exactly equal to its predecessor's.
+// Whole-tile queue where most foreground voxels are queued: under 4 and
+// binary 6 every foreground voxel, under 18 every foreground run member (a
+// column link leaves two in-plane corners uncovered), and for continuous
+// links every run member whose value is not exactly equal to its
+// predecessor's.  Multi-label 6-connected tiles queue their run starts and
+// the run members that still need a face union, about as many as under 26.
 template <int MODE, int NB>
 __host__ __device__ constexpr bool full_queue() {
-  return NB == 6 || NB == 4 || NB == 18 || MODE == M_CONT;
+  return (NB == 6 && MODE == M_BINARY) || NB == 4 || NB == 18 || MODE == M_CONT;
 }
 
 // LDS of one tile (declared once per kernel, so kernels that dispatch to

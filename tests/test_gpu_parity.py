@@ -109,6 +109,16 @@ def test_random_tile_crossing(gpu):
     _check(cc3d, arr, kw, f"big#{i}")
 
 
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64, np.float32])
+def test_queue_overflow_6conn(gpu, dtype):
+  """Noise with three labels: most voxels start a run, so a tile queues more
+  voxels than the work queue holds and the rest are unioned after it."""
+  import cc3d
+  rng = np.random.default_rng(66)
+  arr = rng.integers(0, 3, size=(96, 40, 24)).astype(dtype)
+  _check(cc3d, np.asfortranarray(arr), dict(connectivity=6), f"overflow6 {np.dtype(dtype).name}")
+
+
 @pytest.mark.parametrize("conn", [26, 18, 6])
 @pytest.mark.parametrize("order", ["C", "F"])
 def test_voronoi_multilabel(gpu, conn, order):
