@@ -168,20 +168,22 @@ struct VolGeo {
   uint32_t key0;
 };
 
-__device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
+__device__ inline VolGeo batch_geo(const BatchVol& v) {
   VolGeo g;
-  if (p.nvol == 0) {
-    g.sx = p.sx; g.sy = p.sy; g.sz = p.sz; g.osx = p.osx;
-    g.ntx = p.ntx; g.nty = p.nty; g.ntz = p.ntz;
-    g.tile0 = 0;
-    g.key0 = 0;
-  } else {
-    const BatchVol& v = p.vols[find_vol(p.vtile0, p.nvol, slot)];
-    g.sx = v.sx; g.sy = v.sy; g.sz = v.sz; g.osx = v.osx;
-    g.ntx = v.ntx; g.nty = v.nty; g.ntz = v.ntz;
-    g.tile0 = v.tile0;
-    g.key0 = v.key0;
-  }
+  g.sx = v.sx; g.sy = v.sy; g.sz = v.sz; g.osx = v.osx;
+  g.ntx = v.ntx; g.nty = v.nty; g.ntz = v.ntz;
+  g.tile0 = v.tile0;
+  g.key0 = v.key0;
+  return g;
+}
+
+__device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
+  if (p.nvol != 0) return batch_geo(p.vols[find_vol(p.vtile0, p.nvol, slot)]);
+  VolGeo g;
+  g.sx = p.sx; g.sy = p.sy; g.sz = p.sz; g.osx = p.osx;
+  g.ntx = p.ntx; g.nty = p.nty; g.ntz = p.ntz;
+  g.tile0 = 0;
+  g.key0 = 0;
   return g;
 }
 
@@ -474,18 +476,15 @@ __device__ inline uint32_t root_label(const GlobalParams& p, uint32_t r, uint32_
   return p.prefix[wd] - base + __popc(p.bits[wd] & ((1u << (r & 31)) - 1u)) + 1u;
 }
 
-template <class OUT, class G>
-__device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_t tile, const VolGeo& v,
-                                     OUT* __restrict__ out, uint32_t* lds_labels) {
+// Root indices of thread t's 8 J entries of tile `slot` (one vector load),
+// and the tile's label table: entry 0 background, entry 1 + i the label of
+// local root i.
+template <class G>
+__device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t base, uint32_t (&j)[8],
+                                uint32_t* lab) {
   static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
   const int t = threadIdx.x;
-  const int64_t tz = tile / (v.ntx * v.nty);
-  const int64_t tr = tile - tz * v.ntx * v.nty;
-  const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
-  const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
-  const uint32_t* __restrict__ LT = p.RL + slot * G::TV;
   const uint32_t R = p.TR[slot];
-  uint32_t j[8];
   if (R <= 255) {
     const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + slot * G::TV)[t];
 #pragma unroll
@@ -496,16 +495,26 @@ __device__ inline void finalize_tile(const GlobalParams& p, int64_t slot, int64_
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
-  lds_labels[0] = 0u;  // index 0: background
-  const uint32_t base = p.nvol ? p.prefix[v.key0 >> 5] : 0u;
-  for (uint32_t i = t; i < R; i += WG) lds_labels[i + 1] = root_label(p, LT[i], base);
-  __syncthreads();
+  const uint32_t* __restrict__ LT = p.RL + slot * G::TV;
+  lab[0] = 0u;
+  for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
+}
+
+// Thread t's 8 output voxels of tile `tile` of volume v (after a barrier
+// behind fin_load).
+template <class OUT, class G>
+__device__ inline void fin_store(const VolGeo& v, int64_t tile, OUT* __restrict__ out, const uint32_t (&j)[8],
+                                 const uint32_t* lab) {
+  const int t = threadIdx.x;
+  const int64_t tz = tile / (v.ntx * v.nty);
+  const int64_t tr = tile - tz * v.ntx * v.nty;
+  const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    int lx, ly, lz;
-    G::entry_voxel(8 * t + k, lx, ly, lz);
-    const int64_t gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lds_labels[j[k]];
+    int vx, vy, vz;
+    G::entry_voxel(8 * t + k, vx, vy, vz);
+    const int64_t gx = tx * G::TX + vx, gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
+    if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[j[k]];
   }
 }
 
@@ -516,63 +525,53 @@ constexpr int FIN_TILES = 2;
 
 template <class OUT, class G>
 __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
-  static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
   __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
-  const int t = threadIdx.x;
   const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
   const int64_t first = xcd_tile(blockIdx.x, ngroups) * FIN_TILES;
   const VolGeo v = vol_geo(p, 0);
   uint32_t j[FIN_TILES][8];
 #pragma unroll
   for (int n = 0; n < FIN_TILES; n++) {
-    const int64_t tile = first + n;
-    if (tile >= p.ntiles) break;
-    const uint32_t R = p.TR[tile];
-    if (R <= 255) {
-      const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + tile * G::TV)[t];
-#pragma unroll
-      for (int k = 0; k < 8; k++) j[n][k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
-    } else {
-      const uint4 w = reinterpret_cast<const uint4*>(p.J16 + tile * G::TV)[t];
-      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int k = 0; k < 8; k++) j[n][k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
-    }
-    const uint32_t* __restrict__ LT = p.RL + tile * G::TV;
-    lab[n][0] = 0u;
-    for (uint32_t i = t; i < R; i += WG) lab[n][i + 1] = root_label(p, LT[i], 0u);
+    if (first + n >= p.ntiles) break;
+    fin_load<G>(p, first + n, 0u, j[n], lab[n]);
   }
   __syncthreads();
 #pragma unroll
   for (int n = 0; n < FIN_TILES; n++) {
-    const int64_t tile = first + n;
-    if (tile >= p.ntiles) break;
-    const int64_t tz = tile / (v.ntx * v.nty);
-    const int64_t tr = tile - tz * v.ntx * v.nty;
-    const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      int vx, vy, vz;
-      G::entry_voxel(8 * t + k, vx, vy, vz);
-      const int64_t gx = tx * G::TX + vx, gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
-      if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[n][j[n][k]];
-    }
+    if (first + n >= p.ntiles) break;
+    fin_store<OUT, G>(v, first + n, out, j[n], lab[n]);
   }
 }
 
-// Batch: each tile writes its own volume's output in that volume's dtype.
+// Batch: FIN_TILES consecutive tiles per workgroup, each writing its own
+// volume's output in that volume's dtype, numbered from the prefix at the
+// volume's first key word.
 template <class G>
 __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
-  __shared__ uint32_t lab[G::TV + 1];
-  const int64_t slot = xcd_tile(blockIdx.x, p.ntiles);
-  const BatchVol& bv = p.vols[find_vol(p.vtile0, p.nvol, slot)];
-  const VolGeo v = vol_geo(p, slot);
-  if (bv.out_dtype == DT_U16)
-    finalize_tile<uint16_t, G>(p, slot, slot - v.tile0, v, static_cast<uint16_t*>(bv.out), lab);
-  else if (bv.out_dtype == DT_U64)
-    finalize_tile<uint64_t, G>(p, slot, slot - v.tile0, v, static_cast<uint64_t*>(bv.out), lab);
-  else
-    finalize_tile<uint32_t, G>(p, slot, slot - v.tile0, v, static_cast<uint32_t*>(bv.out), lab);
+  __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
+  const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
+  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FIN_TILES;
+  uint32_t j[FIN_TILES][8];
+  int vi[FIN_TILES];
+#pragma unroll
+  for (int n = 0; n < FIN_TILES; n++) {
+    const int64_t slot = first + n;
+    if (slot >= p.ntiles) break;
+    vi[n] = find_vol(p.vtile0, p.nvol, slot);
+    fin_load<G>(p, slot, p.prefix[p.vols[vi[n]].key0 >> 5], j[n], lab[n]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < FIN_TILES; n++) {
+    const int64_t slot = first + n;
+    if (slot >= p.ntiles) break;
+    const BatchVol& bv = p.vols[vi[n]];
+    const VolGeo v = batch_geo(bv);
+    const int64_t tile = slot - bv.tile0;
+    if (bv.out_dtype == DT_U16) fin_store<uint16_t, G>(v, tile, static_cast<uint16_t*>(bv.out), j[n], lab[n]);
+    else if (bv.out_dtype == DT_U64) fin_store<uint64_t, G>(v, tile, static_cast<uint64_t*>(bv.out), j[n], lab[n]);
+    else fin_store<uint32_t, G>(v, tile, static_cast<uint32_t*>(bv.out), j[n], lab[n]);
+  }
 }
 
 // Batch: per-volume counters from the per-tile records (one block per volume).
@@ -1876,7 +1875,8 @@ int cc3d_amd_batch_finish(cc3d_amd_batch* b) {
   if (!b) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
   if (!b->outputs_set) return fail(CC3D_AMD_ERR_INVALID, "batch outputs are not set");
   ProfScope ps_("k_finalize", b->stream);
-  hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)b->ntiles), dim3(WG), 0, b->stream, b->gp);
+  hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)((b->ntiles + FIN_TILES - 1) / FIN_TILES)), dim3(WG), 0,
+                     b->stream, b->gp);
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
 }
