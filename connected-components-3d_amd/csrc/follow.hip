@@ -102,7 +102,10 @@ __device__ inline void load_seg(const L* p, L (&v)[ST_SEG]) {
   }
 }
 
-// Per-label statistics, W: bit 0 bounding boxes, bit 1 coordinate sums.
+// Per-label statistics, W: bit 0 bounding boxes, bit 1 coordinate sums,
+// bit 2 counts are final already (a counting pass ran before): no count
+// atomics, and a label whose count in the brick equals its total lies in
+// this brick only, so its boxes and sums are written with plain stores.
 // Each thread splits its segment into runs of one label; each run is added
 // to the brick's LDS table (count, min/max and coordinate sums relative to
 // the brick origin; along a row only the fastest coordinate varies), and
@@ -114,7 +117,7 @@ __device__ inline void load_seg(const L* p, L (&v)[ST_SEG]) {
 template <class L, int W>
 __global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t* counts, uint32_t* bbox,
                                               unsigned long long* sums) {
-  constexpr bool BB = (W & 1) != 0, SM = (W & 2) != 0;
+  constexpr bool BB = (W & 1) != 0, SM = (W & 2) != 0, CF = (W & 4) != 0;
   constexpr unsigned long long EMPTY = ~0ull;
   __shared__ unsigned long long tkey[ST_SLOTS];
   __shared__ uint32_t tcnt[ST_SLOTS];
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t*
             if constexpr (SM) atomicAdd(tsm + 3 * s + k, ls[k]);
           }
         } else {
-          atomicAdd(counts + key, c);
+          if constexpr (!CF) atomicAdd(counts + key, c);
 #pragma unroll
           for (int k = 0; k < 3; k++) {
             const int a = g.p[k];
@@ -230,16 +233,28 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t*
         }
       } else {
         const unsigned long long key = tkey[s];
-        atomicAdd(counts + key, c);
+        if constexpr (!CF) atomicAdd(counts + key, c);
+        const bool only = CF && counts[key] == c;  // the label's every voxel is in this slot
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           const int a = g.p[k];
           if (a >= nd) continue;
-          if constexpr (BB) {
-            atomicMin(bbox + 2 * nd * key + 2 * a, (uint32_t)(o[k] + tmn[3 * s + k]));
-            atomicMax(bbox + 2 * nd * key + 2 * a + 1, (uint32_t)(o[k] + tmx[3 * s + k]));
+          const uint32_t lo = (uint32_t)(o[k] + (BB ? tmn[3 * s + k] : 0u));
+          const uint32_t hi = (uint32_t)(o[k] + (BB ? tmx[3 * s + k] : 0u));
+          const unsigned long long sm = (unsigned long long)c * o[k] + (SM ? tsm[3 * s + k] : 0u);
+          if (only) {
+            if constexpr (BB) {
+              bbox[2 * nd * key + 2 * a] = lo;
+              bbox[2 * nd * key + 2 * a + 1] = hi;
+            }
+            if constexpr (SM) sums[nd * key + a] = sm;
+          } else {
+            if constexpr (BB) {
+              atomicMin(bbox + 2 * nd * key + 2 * a, lo);
+              atomicMax(bbox + 2 * nd * key + 2 * a + 1, hi);
+            }
+            if constexpr (SM) atomicAdd(sums + nd * key + a, sm);
           }
-          if constexpr (SM) atomicAdd(sums + nd * key + a, (unsigned long long)c * o[k] + tsm[3 * s + k]);
         }
       }
       reset(s);
@@ -247,7 +262,7 @@ __global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t*
     __syncthreads();
   }
   if (t == 0 && seen0) {
-    atomicAdd(counts, c0);
+    if constexpr (!CF) atomicAdd(counts, c0);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int a = g.p[k];
@@ -320,14 +335,24 @@ static void launch_stats_w(const void* labels, const StatGeo& g, uint32_t* count
                      sums);
 }
 
+// With boxes or sums and fewer than 2^32 voxels (no count can wrap), counts
+// come first in a pass of their own, and the second pass writes the labels
+// that lie in a single brick (most labels of a noisy volume) without atomics.
 template <class L>
-static void launch_stats(int w, const void* labels, const StatGeo& g, uint32_t* counts, uint32_t* bbox,
+static void launch_stats(int w, int64_t n, const void* labels, const StatGeo& g, uint32_t* counts, uint32_t* bbox,
                          unsigned long long* sums, hipStream_t s) {
+  if (w != 0 && n < (1ll << 32)) {
+    launch_stats_w<L, 0>(labels, g, counts, bbox, sums, s);
+    w |= 4;
+  }
   switch (w) {
     case 0: launch_stats_w<L, 0>(labels, g, counts, bbox, sums, s); break;
     case 1: launch_stats_w<L, 1>(labels, g, counts, bbox, sums, s); break;
     case 2: launch_stats_w<L, 2>(labels, g, counts, bbox, sums, s); break;
-    default: launch_stats_w<L, 3>(labels, g, counts, bbox, sums, s); break;
+    case 3: launch_stats_w<L, 3>(labels, g, counts, bbox, sums, s); break;
+    case 5: launch_stats_w<L, 5>(labels, g, counts, bbox, sums, s); break;
+    case 6: launch_stats_w<L, 6>(labels, g, counts, bbox, sums, s); break;
+    default: launch_stats_w<L, 7>(labels, g, counts, bbox, sums, s); break;
   }
 }
 
@@ -413,10 +438,10 @@ int cc3d_amd_statistics(const void* labels, int dtype, int64_t s0, int64_t s1, i
     g.nbricks = g.nbx * g.nby * ((g.m2 + (1ll << g.zl) - 1) >> g.zl);
     const int w = (bbox ? 1 : 0) | (sums ? 2 : 0);
     switch (dtype) {
-      case DT_U8: launch_stats<uint8_t>(w, labels, g, counts, bbox, sums, s); break;
-      case DT_U16: launch_stats<uint16_t>(w, labels, g, counts, bbox, sums, s); break;
-      case DT_U32: launch_stats<uint32_t>(w, labels, g, counts, bbox, sums, s); break;
-      default: launch_stats<uint64_t>(w, labels, g, counts, bbox, sums, s); break;
+      case DT_U8: launch_stats<uint8_t>(w, n, labels, g, counts, bbox, sums, s); break;
+      case DT_U16: launch_stats<uint16_t>(w, n, labels, g, counts, bbox, sums, s); break;
+      case DT_U32: launch_stats<uint32_t>(w, n, labels, g, counts, bbox, sums, s); break;
+      default: launch_stats<uint64_t>(w, n, labels, g, counts, bbox, sums, s); break;
     }
   }
   if (centroids)
