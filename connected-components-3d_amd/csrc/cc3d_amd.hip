@@ -351,7 +351,7 @@ __global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
     const uint32_t R = p.TR[tile];
     for (uint32_t j = sub; j < R; j += TILE_LANES) {
       const uint32_t g = p.RL[tile * G::TV + j];
-      const uint32_t r = gfind(p.GP, g);
+      const uint32_t r = gfind_settled(p.GP, g);
       if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
       p.RL[tile * G::TV + j] = r;  // from here on RL holds each local root's global root
     }

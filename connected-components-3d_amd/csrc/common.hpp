@@ -364,6 +364,21 @@ __device__ inline uint32_t gfind(uint32_t* GP, uint32_t a) {
   }
   return a;
 }
+// Find once every union is made (after the last union kernel): the forest now
+// only gets compressed, so every value a node's parent takes is one of its
+// ancestors and roots stay fixed.  A stale cached parent still leads to the
+// root, so plain loads (which may hit in L2) replace the coherent ones.
+__device__ inline uint32_t gfind_settled(uint32_t* GP, uint32_t a) {
+  uint32_t p = GP[a];
+  while (p != a) {
+    const uint32_t gp = GP[p];
+    if (gp == p) return p;
+    atomicMin(GP + a, gp);
+    a = gp;
+    p = GP[a];
+  }
+  return a;
+}
 __device__ inline void gunion(uint32_t* GP, uint32_t a, uint32_t b) {
   while (true) {
     a = gfind(GP, a);
