@@ -7,7 +7,7 @@ signatures and results, computed by hand-written HIP kernels for gfx950
 """
 from ._batch import LabelBatch, connected_components_batch
 from ._core import DimensionError, connected_components, estimate_provisional_labels
-from ._follow import dust, statistics
+from ._follow import dust, largest_k, statistics
 from ._serve import LabelServer, Request
 
 __version__ = "4.0.0+mi355x.0.1"
@@ -17,6 +17,7 @@ __all__ = [
   "estimate_provisional_labels",
   "statistics",
   "dust",
+  "largest_k",
   "DimensionError",
   "connected_components_batch",
   "LabelBatch",

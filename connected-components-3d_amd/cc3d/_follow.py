@@ -1,4 +1,5 @@
-"""statistics and dust on the GPU (cc3d/fastcc3d.pyx:681-938, cc3d/__init__.py:71-155)."""
+"""statistics, dust and largest_k on the GPU (cc3d/fastcc3d.pyx:681-938,
+cc3d/__init__.py:71-155, :199-279)."""
 from __future__ import annotations
 
 import ctypes
@@ -22,6 +23,8 @@ def _bind():
     lib.cc3d_amd_dust_apply.restype = ci
     lib.cc3d_amd_dust_apply.argtypes = [vp, ci, vp, ci, i64, vp, u64, ctypes.c_double, ctypes.c_double,
                                         ci, ci, vp, vp, vp]
+    lib.cc3d_amd_keep_labels.restype = ci
+    lib.cc3d_amd_keep_labels.argtypes = [vp, ci, i64, vp, i64, u64, vp, vp, ci, vp]
     lib.cc3d_amd_follow_last_error.restype = ctypes.c_char_p
     lib._follow_bound = True
   return lib
@@ -52,13 +55,10 @@ def _device_stats(buf, code, s0, s1, s2, ndim, f_order, N, bbox=True, centroids=
   return counts, bb, cen
 
 
-def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
-  """Voxel counts, bounding boxes and centroids per label (cc3d/fastcc3d.pyx:681-938).
-
-  Counts are uint32, bounding boxes uint16 (uint32 when a side exceeds 65535)
-  as [min0, max0, min1, max1(, min2, max2)] along the numpy axes, centroids
-  float64 (NaN for absent labels).  Computed on the GPU; returned as numpy.
-  """
+def _label_volume(out_labels):
+  """The checks of cc3d/fastcc3d.pyx:716-748 on a label volume, which is then
+  resident in HBM.  None for an empty volume, else (buf, code, sx, sy, s2,
+  ndim, f_order, N) with N the largest label."""
   vol = _Vol(out_labels)
   shape = list(vol.shape)
   order = "F" if vol.f_contig else "C"
@@ -69,7 +69,7 @@ def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
   sz = shape[2] if ndim > 2 else 1
   voxels = sx * sy * sz
   if voxels == 0:
-    return {"voxel_counts": None, "bounding_boxes": None, "centroids": None}
+    return None
   npdt = vol.np_dtype
   if npdt == np.bool_:
     npdt = np.dtype(np.uint8)
@@ -97,12 +97,22 @@ def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
       f"Statistics can only be computed on volumes containing labels with values >= 0. Min: {int(smin.value)}"
     )
   # memory layout: the 2D/3D array in its own order; axes as numpy sees them
-  s2 = sz if ndim > 2 else 1
-  if order == "F":
-    f_order = 1
-  else:
-    f_order = 0
-  counts, bbox, cen = _device_stats(buf, code, sx, sy, s2, ndim, f_order, N)
+  return buf, code, sx, sy, sz, ndim, int(order == "F"), N
+
+
+def statistics(out_labels, no_slice_conversion: bool = False) -> dict:
+  """Voxel counts, bounding boxes and centroids per label (cc3d/fastcc3d.pyx:681-938).
+
+  Counts are uint32, bounding boxes uint16 (uint32 when a side exceeds 65535)
+  as [min0, max0, min1, max1(, min2, max2)] along the numpy axes, centroids
+  float64 (NaN for absent labels).  Computed on the GPU; returned as numpy.
+  """
+  lv = _label_volume(out_labels)
+  if lv is None:
+    return {"voxel_counts": None, "bounding_boxes": None, "centroids": None}
+  buf, code, sx, sy, sz, ndim, f_order, N = lv
+  voxels = sx * sy * sz
+  counts, bbox, cen = _device_stats(buf, code, sx, sy, sz, ndim, f_order, N)
   nd = 2 if ndim == 2 else 3
   counts = counts.cpu().numpy().view(np.uint32)
   bb = bbox.cpu().numpy().view(np.uint32)
@@ -203,3 +213,58 @@ def dust(img, threshold: Union[int, float, tuple, list], connectivity: int = 26,
 
 def _strides_of(a: np.ndarray):
   return tuple(s // a.itemsize for s in a.strides)
+
+
+def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0, return_N: bool = False,
+              binary_image: bool = False, precomputed_ccl: bool = False):
+  """The k largest connected components (cc3d/__init__.py:199-279), numbered
+  1..k in ascending size order as the reference does without fastremap.
+
+  Labelling, voxel counts and the renumbering pass run on the GPU.  The
+  choice of the k labels is the reference's own numpy code (argpartition and
+  a stable sort by count) on the same uint32 counts, so ties break the same
+  way.  k == 1 returns the boolean mask of the largest component.
+  """
+  assert k >= 0
+  torch = _torch()
+  if _is_torch(img):
+    raise TypeError("largest_k takes numpy arrays (as the reference does)")
+  order = "C" if img.flags.c_contiguous else "F"
+  if k == 0:
+    return np.zeros(img.shape, dtype=np.uint16, order=order)
+  if precomputed_ccl:
+    cc_labels = np.copy(img, order="F")
+    N = np.max(cc_labels)
+  else:
+    cc_labels, N = connected_components(img, connectivity=connectivity, return_N=True, delta=delta,
+                                        binary_image=bool(binary_image))
+  if N <= k:
+    return (cc_labels, N) if return_N else cc_labels
+
+  lv = _label_volume(cc_labels)
+  buf, code, sx, sy, sz, ndim, f_order, NL = lv
+  counts, _, _ = _device_stats(buf, code, sx, sy, sz, ndim, f_order, NL, bbox=False, centroids=False)
+  cts = counts.cpu().numpy().view(np.uint32)
+  if k == 1:
+    keep = [int(np.argmax(cts[1:])) + 1]
+    out_np = np.dtype(np.bool_)
+  else:
+    preserve = np.argpartition(cts[1:], len(cts) - k - 1)[-k:]
+    preserve += 1
+    keep = list(map(int, sorted(preserve, key=lambda label: cts[label])))
+    out_np = np.dtype(cc_labels.dtype)
+  voxels = sx * sy * sz
+  keep_t = torch.tensor(keep, dtype=torch.int64).to(buf.device)
+  lut = torch.empty(NL + 1, dtype=torch.int32, device=buf.device)
+  out_t = torch.empty(voxels * out_np.itemsize, dtype=torch.uint8, device=buf.device)
+  _fcheck(_bind().cc3d_amd_keep_labels(
+    ctypes.c_void_p(buf.data_ptr()), code, voxels, ctypes.c_void_p(keep_t.data_ptr()), len(keep), NL,
+    ctypes.c_void_p(lut.data_ptr()), ctypes.c_void_p(out_t.data_ptr()), _NP_TO_CODE[out_np], _stream()))
+  host = out_t.cpu().numpy()
+  cc_order = "C" if cc_labels.flags.c_contiguous else "F"
+  cc_out = np.zeros(cc_labels.shape, dtype=out_np, order=cc_order)
+  flat = np.ravel(cc_out, order="K")
+  flat[:] = host.view(flat.dtype)
+  if return_N:
+    return cc_out, (1 if k == 1 else len(keep))
+  return cc_out

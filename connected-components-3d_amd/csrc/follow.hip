@@ -309,6 +309,26 @@ __global__ __launch_bounds__(WG) void k_dust_apply(T* img, const L* lab, const u
     if (erase[(uint64_t)lab[i]]) img[i] = T(0);
 }
 
+// largest_k renumbering: lut[l] = output label of label l (0: dropped).
+__global__ void k_lut_clear(uint32_t* lut, int64_t n1) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n1; i += (int64_t)gridDim.x * WG) lut[i] = 0u;
+}
+
+__global__ void k_lut_set(uint32_t* lut, const uint64_t* keep, int64_t k, uint64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < k; i += (int64_t)gridDim.x * WG) {
+    const uint64_t l = keep[i];
+    if (l <= N) lut[l] = (uint32_t)(i + 1);
+  }
+}
+
+template <class L, class O>
+__global__ __launch_bounds__(WG) void k_keep_apply(O* out, const L* lab, const uint32_t* lut, uint64_t N, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG) {
+    const uint64_t l = (uint64_t)lab[i];
+    out[i] = O(l <= N ? lut[l] : 0u);
+  }
+}
+
 static thread_local std::string f_err;
 
 static int ffail(int code, const char* msg) {
@@ -481,6 +501,44 @@ int cc3d_amd_dust_apply(void* img, int img_dtype, const void* labels, int label_
 #undef CC3D_DL
 #undef CC3D_DA
   if (hipGetLastError() != hipSuccess) return ffail(CC3D_AMD_ERR_HIP, "dust launch failed");
+  return CC3D_AMD_OK;
+}
+
+// largest_k's renumbering without fastremap (cc3d/__init__.py:266-275):
+// out[v] = i + 1 where labels[v] == keep[i], else 0.  keep: device uint64[k];
+// lut_scratch: device uint32[N+1]; labels are <= N.  out_dtype DT_U8 with
+// k == 1 gives the boolean mask of the k == 1 branch (:250-254).
+int cc3d_amd_keep_labels(const void* labels, int label_dtype, int64_t voxels, const uint64_t* keep, int64_t k,
+                         uint64_t N, uint32_t* lut_scratch, void* out, int out_dtype, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!labels || !out || !lut_scratch || (k > 0 && !keep) || k < 0 || voxels < 0)
+    return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: bad argument");
+  if (k > 0xFFFFFFFFll) return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: k must fit 32 bits");
+  const int64_t n1 = (int64_t)N + 1;
+  hipLaunchKernelGGL(k_lut_clear, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, lut_scratch, n1);
+  if (k > 0) hipLaunchKernelGGL(k_lut_set, dim3(fgrid(k, WG, 8192)), dim3(WG), 0, s, lut_scratch, keep, k, N);
+  const dim3 g(fgrid(voxels, WG, 65536));
+#define CC3D_KA(L, O) hipLaunchKernelGGL((k_keep_apply<L, O>), g, dim3(WG), 0, s, (O*)out, (const L*)labels, lut_scratch, N, voxels)
+#define CC3D_KO(L)                                                   \
+  switch (out_dtype) {                                               \
+    case DT_U8: CC3D_KA(L, uint8_t); break;                          \
+    case DT_U16: CC3D_KA(L, uint16_t); break;                        \
+    case DT_U32: CC3D_KA(L, uint32_t); break;                        \
+    case DT_U64: CC3D_KA(L, uint64_t); break;                        \
+    default: return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: output must be u8/u16/u32/u64"); \
+  }
+  if (voxels > 0) {
+    switch (label_dtype) {
+      case DT_U8: CC3D_KO(uint8_t); break;
+      case DT_U16: CC3D_KO(uint16_t); break;
+      case DT_U32: CC3D_KO(uint32_t); break;
+      case DT_U64: CC3D_KO(uint64_t); break;
+      default: return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: labels must be unsigned integers");
+    }
+  }
+#undef CC3D_KO
+#undef CC3D_KA
+  if (hipGetLastError() != hipSuccess) return ffail(CC3D_AMD_ERR_HIP, "keep_labels launch failed");
   return CC3D_AMD_OK;
 }
 

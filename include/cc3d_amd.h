@@ -169,6 +169,14 @@ int cc3d_amd_dust_apply(void* img, int img_dtype, const void* labels, int label_
                         int64_t voxels, const uint32_t* counts, uint64_t N, double lo, double hi,
                         int ranged, int invert, uint8_t* erase_scratch,
                         unsigned long long* n_masked, void* stream);
+/* The renumbering step of cc3d.largest_k without fastremap
+ * (cc3d/__init__.py:250-275): out[v] = i + 1 where labels[v] == keep[i], else
+ * 0 (uint8 output with k == 1: the boolean mask).  keep: device uint64[k], the
+ * labels to keep in output order; lut_scratch: device uint32[N+1]; every label
+ * is <= N. */
+int cc3d_amd_keep_labels(const void* labels, int label_dtype, int64_t voxels, const uint64_t* keep,
+                         int64_t k, uint64_t N, uint32_t* lut_scratch, void* out, int out_dtype,
+                         void* stream);
 const char* cc3d_amd_follow_last_error(void);
 
 /* Per-kernel timing with HIP events recorded on the launch stream.  No

@@ -158,3 +158,78 @@ def test_dust_reference_known_answer(gpu):
     assert list(np.unique(rec)) == [0, 2] and N == 1
     rec, N = cc3d.dust(labels, threshold=26, connectivity=conn, invert=True, return_N=True)
     assert list(np.unique(rec)) == [0, 1] and N == 1
+
+
+def _largest_k_restated(img, k, connectivity=26, delta=0, return_N=False, binary_image=False,
+                        precomputed_ccl=False):
+  """cc3d/__init__.py:199-279 without fastremap (absent here, as in a plain
+  install), with the oracle's labels and numpy's counts."""
+  order = "C" if img.flags.c_contiguous else "F"
+  if k == 0:
+    return np.zeros(img.shape, dtype=np.uint16, order=order)
+  if precomputed_ccl:
+    cc = np.copy(img, order="F")
+    N = np.max(cc)
+  else:
+    cc, N = oracle.connected_components(img, connectivity=connectivity, return_N=True, delta=delta,
+                                        binary_image=bool(binary_image))
+  if N <= k:
+    return (cc, N) if return_N else cc
+  cts = np.bincount(cc.ravel().astype(np.int64), minlength=int(N) + 1).astype(np.uint32)
+  if k == 1:
+    out = cc == (np.argmax(cts[1:]) + 1)
+    return (out, 1) if return_N else out
+  preserve = np.argpartition(cts[1:], len(cts) - k - 1)[-k:] + 1
+  keep = list(map(int, sorted(preserve, key=lambda label: cts[label])))
+  out = np.zeros(cc.shape, dtype=cc.dtype, order="C" if cc.flags.c_contiguous else "F")
+  for i, label in enumerate(keep):
+    out[cc == label] = i + 1
+  return (out, len(keep)) if return_N else out
+
+
+def _same_array(a, b):
+  return (a.dtype == b.dtype and a.shape == b.shape and a.flags.c_contiguous == b.flags.c_contiguous
+          and a.flags.f_contiguous == b.flags.f_contiguous and np.array_equal(a, b))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5, 50])
+@pytest.mark.parametrize("order,connectivity", [("C", 26), ("F", 6)])
+def test_largest_k_matches_restatement(gpu, k, order, connectivity):
+  """The reference's own case (automated_test.py:1286-1321: uint8 noise with 5
+  values, 100^3), checked exactly, ties included."""
+  import cc3d
+  rng = np.random.default_rng(1286)
+  img = rng.integers(0, 5, size=(100, 100, 100)).astype(np.uint8)
+  img = np.asfortranarray(img) if order == "F" else np.ascontiguousarray(img)
+  got = cc3d.largest_k(img, k=k, connectivity=connectivity, return_N=True)
+  exp = _largest_k_restated(img, k, connectivity=connectivity, return_N=True)
+  if k == 0:
+    assert _same_array(got, exp)
+    return
+  assert _same_array(got[0], exp[0]) and int(got[1]) == int(exp[1])
+
+
+def test_largest_k_variants(gpu):
+  """precomputed labels (signed, C order; also N <= k), binary and continuous
+  inputs, and a 2D image."""
+  import cc3d
+  import synth
+  vor = synth.voronoi((96, 80, 40), 60, seed=7, id_range=(1, 1000))
+  cc = oracle.connected_components(vor, connectivity=26).astype(np.int32)
+  cc = np.ascontiguousarray(cc)
+  cases = [
+    (cc, dict(k=7, precomputed_ccl=True, return_N=True)),
+    (cc, dict(k=1, precomputed_ccl=True)),
+    (cc, dict(k=10 ** 6, precomputed_ccl=True, return_N=True)),
+    (vor > 500, dict(k=3, binary_image=True, connectivity=18)),
+    (synth.tones((64, 64, 32), 40, seed=3), dict(k=4, delta=20, connectivity=26)),
+    (np.random.default_rng(9).integers(0, 3, size=(300, 200)).astype(np.uint16), dict(k=6, connectivity=8)),
+  ]
+  for i, (img, kw) in enumerate(cases):
+    print("largest_k case", i, img.shape, img.dtype, kw, flush=True)
+    got = cc3d.largest_k(img, **kw)
+    exp = _largest_k_restated(img, **kw)
+    if kw.get("return_N"):
+      assert _same_array(got[0], exp[0]) and int(got[1]) == int(exp[1]), i
+    else:
+      assert _same_array(got, exp), i
