@@ -235,13 +235,33 @@ def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0,
   if precomputed_ccl:
     cc_labels = np.copy(img, order="F")
     N = np.max(cc_labels)
+    if N <= k:
+      return (cc_labels, N) if return_N else cc_labels
+    cc_shape, cc_dtype, cc_c = cc_labels.shape, cc_labels.dtype, cc_labels.flags.c_contiguous
+    lv = _label_volume(cc_labels)
+  elif img.size and (img.flags.c_contiguous or img.flags.f_contiguous) and img.dtype != np.float16:
+    # the labels stay in HBM: one copy of the image in, one of the result out
+    _require_gpu()
+    img_dev = _Vol(img).device_buffer("F" if img.flags.f_contiguous else "C")
+    tview = torch.as_strided(img_dev, img.shape, _strides_of(img))
+    if img.dtype == np.bool_:
+      tview = tview.view(torch.bool)  # keeps the reference's bool => binary rule
+    cc_t, N = connected_components(tview, connectivity=connectivity, return_N=True, delta=delta,
+                                   binary_image=bool(binary_image))
+    if N <= k:
+      cc_labels = cc_t.cpu().numpy()  # the input's shape and memory order
+      return (cc_labels, N) if return_N else cc_labels
+    cc_shape, cc_c = img.shape, img.flags.c_contiguous
+    cc_dtype = np.dtype({torch.uint16: np.uint16, torch.uint32: np.uint32, torch.uint64: np.uint64}[cc_t.dtype])
+    lv = _label_volume(cc_t)
   else:
     cc_labels, N = connected_components(img, connectivity=connectivity, return_N=True, delta=delta,
                                         binary_image=bool(binary_image))
-  if N <= k:
-    return (cc_labels, N) if return_N else cc_labels
+    if N <= k:
+      return (cc_labels, N) if return_N else cc_labels
+    cc_shape, cc_dtype, cc_c = cc_labels.shape, cc_labels.dtype, cc_labels.flags.c_contiguous
+    lv = _label_volume(cc_labels)
 
-  lv = _label_volume(cc_labels)
   buf, code, sx, sy, sz, ndim, f_order, NL = lv
   counts, _, _ = _device_stats(buf, code, sx, sy, sz, ndim, f_order, NL, bbox=False, centroids=False)
   cts = counts.cpu().numpy().view(np.uint32)
@@ -252,7 +272,7 @@ def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0,
     preserve = np.argpartition(cts[1:], len(cts) - k - 1)[-k:]
     preserve += 1
     keep = list(map(int, sorted(preserve, key=lambda label: cts[label])))
-    out_np = np.dtype(cc_labels.dtype)
+    out_np = np.dtype(cc_dtype)
   voxels = sx * sy * sz
   keep_t = torch.tensor(keep, dtype=torch.int64).to(buf.device)
   lut = torch.empty(NL + 1, dtype=torch.int32, device=buf.device)
@@ -261,8 +281,7 @@ def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0,
     ctypes.c_void_p(buf.data_ptr()), code, voxels, ctypes.c_void_p(keep_t.data_ptr()), len(keep), NL,
     ctypes.c_void_p(lut.data_ptr()), ctypes.c_void_p(out_t.data_ptr()), _NP_TO_CODE[out_np], _stream()))
   host = out_t.cpu().numpy()
-  cc_order = "C" if cc_labels.flags.c_contiguous else "F"
-  cc_out = np.zeros(cc_labels.shape, dtype=out_np, order=cc_order)
+  cc_out = np.zeros(cc_shape, dtype=out_np, order="C" if cc_c else "F")
   flat = np.ravel(cc_out, order="K")
   flat[:] = host.view(flat.dtype)
   if return_N:

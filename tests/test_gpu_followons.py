@@ -211,7 +211,8 @@ def test_largest_k_matches_restatement(gpu, k, order, connectivity):
 
 def test_largest_k_variants(gpu):
   """precomputed labels (signed, C order; also N <= k), binary and continuous
-  inputs, and a 2D image."""
+  inputs, a 2D image, and inputs labelled through the host path (a strided
+  view, float16)."""
   import cc3d
   import synth
   vor = synth.voronoi((96, 80, 40), 60, seed=7, id_range=(1, 1000))
@@ -224,6 +225,8 @@ def test_largest_k_variants(gpu):
     (vor > 500, dict(k=3, binary_image=True, connectivity=18)),
     (synth.tones((64, 64, 32), 40, seed=3), dict(k=4, delta=20, connectivity=26)),
     (np.random.default_rng(9).integers(0, 3, size=(300, 200)).astype(np.uint16), dict(k=6, connectivity=8)),
+    (np.random.default_rng(10).integers(0, 4, size=(40, 60, 30)).astype(np.int16)[:, ::2, :], dict(k=5)),
+    (np.random.default_rng(11).integers(0, 4, size=(50, 40, 20)).astype(np.float16), dict(k=3)),
   ]
   for i, (img, kw) in enumerate(cases):
     print("largest_k case", i, img.shape, img.dtype, kw, flush=True)
