@@ -179,6 +179,19 @@ int cc3d_amd_keep_labels(const void* labels, int label_dtype, int64_t voxels, co
                          void* stream);
 const char* cc3d_amd_follow_last_error(void);
 
+/* ---- voxel connectivity graph ------------------------------------------ */
+
+/* Replaces extract_voxel_connectivity_graph<T, OUT> (cc3d/cc3d_graphs.hpp:
+ * 192-236, bound at cc3d/fastcc3d.pyx:1021-1176).  labels: device, x fastest,
+ * an unsigned view (dtype 0-3); out: device, (sx*sy*sz) uint32 for
+ * connectivity 26/18, uint8 for 6/8/4.  Bit d is set unless the neighbour in
+ * direction d exists and holds a different label; the bit order is the
+ * reference's (cc3d/fastcc3d.pyx:1036-1064).  4/8 need sz == 1 (status 1,
+ * "sz must be 1 for 2D connectivities."). */
+int cc3d_amd_voxel_connectivity_graph(const void* labels, int dtype, int64_t sx, int64_t sy, int64_t sz,
+                                      int connectivity, void* out, void* stream);
+const char* cc3d_amd_graphs_last_error(void);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  No
  * reference counterpart (measurement support).  on: 0 off, 1 every kernel,
  * 2 only the local labelling kernel.  collect() waits for the events and

@@ -346,3 +346,89 @@ def _special_row(row: int, sx: int, sy: int, flat: np.ndarray, out_dtype):
       out[start + i] = N
       last = val
   return out, N
+
+
+# ---------------------------------------------------------------------------
+# voxel connectivity graph
+# ---------------------------------------------------------------------------
+
+# (dx, dy, dz) of each bit, as published by the reference: the docstring at
+# cc3d/fastcc3d.pyx:1030-1064 and the expected values of automated_test.py:878-950.
+VCG_DIRS_3D = [
+  (+1, 0, 0), (-1, 0, 0), (0, +1, 0), (0, -1, 0), (0, 0, +1), (0, 0, -1),
+  (+1, +1, 0), (-1, +1, 0), (+1, -1, 0), (-1, -1, 0),
+  (+1, 0, +1), (-1, 0, +1), (0, +1, +1), (0, -1, +1), (+1, 0, -1), (-1, 0, -1), (0, +1, -1), (0, -1, -1),
+  (+1, +1, +1), (-1, +1, +1), (+1, -1, +1), (-1, -1, +1), (+1, +1, -1), (-1, +1, -1), (+1, -1, -1), (-1, -1, -1),
+]
+VCG_DIRS_2D = [(+1, 0), (-1, 0), (0, +1), (0, -1), (+1, +1), (-1, +1), (+1, -1), (-1, -1)]
+
+
+def _vcg_bits(lab: np.ndarray, dirs) -> np.ndarray:
+  """Per voxel: bit d cleared where the neighbour at dirs[d] exists and
+  differs (cc3d/cc3d_graphs.hpp:31-182: start from all ones, clear both ends
+  of every differing pair)."""
+  g = np.zeros(lab.shape, dtype=np.uint64)
+  for d, off in enumerate(dirs):
+    keep = np.ones(lab.shape, dtype=bool)
+    src = tuple(slice(max(0, -o), n - max(0, o)) for o, n in zip(off, lab.shape))
+    dst = tuple(slice(max(0, o), n - max(0, -o)) for o, n in zip(off, lab.shape))
+    keep[src] = lab[src] == lab[dst]  # voxel p at src, its neighbour p + off at dst
+    g |= keep.astype(np.uint64) << np.uint64(d)
+  return g
+
+
+def voxel_connectivity_graph(data, connectivity=26, backend: str = "oracle"):
+  """Checker-side cc3d.voxel_connectivity_graph (cc3d/fastcc3d.pyx:1021-1176,
+  cc3d/cc3d_graphs.hpp:192-236)."""
+  data = np.asarray(data)
+  dims = data.ndim
+  if dims not in (1, 2, 3):
+    raise DimensionError("Only 1D, 2D, and 3D arrays supported. Got: " + str(dims))
+  if dims == 2 and connectivity not in (4, 8, 6, 18, 26):
+    raise ValueError("Only 4, 8, and 6, 18, 26 connectivities are supported for 2D images. Got: " + str(connectivity))
+  elif dims != 2 and connectivity not in (6, 18, 26):
+    raise ValueError("Only 6, 18, and 26 connectivities are supported for 3D images. Got: " + str(connectivity))
+  out_dtype = np.uint8 if connectivity in (4, 8, 6) else np.uint32
+  if data.size == 0:
+    return np.zeros(shape=(0,), dtype=out_dtype)
+  data = np.asfortranarray(data)
+  while data.ndim < 3:
+    data = data[..., np.newaxis]
+  if data.dtype in (np.uint64, np.int64):
+    lab = data.view(np.uint64)
+  elif data.dtype in (np.uint32, np.int32):
+    lab = data.view(np.uint32)
+  elif data.dtype in (np.uint16, np.int16):
+    lab = data.view(np.uint16)
+  elif data.dtype in (np.uint8, np.int8, bool):
+    lab = data.view(np.uint8)
+  else:
+    raise TypeError("Type {} not currently supported.".format(data.dtype))
+  sx, sy, sz = lab.shape
+  if backend == "ref":
+    lib = _lib_ref()
+    lib.ref_voxel_connectivity_graph.restype = ctypes.c_int
+    lib.ref_voxel_connectivity_graph.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                                 ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int]
+    flat = np.ravel(lab, order="F")
+    out = np.zeros(flat.size, dtype=out_dtype)
+    rc = lib.ref_voxel_connectivity_graph(flat.ctypes.data, DT_CODE[flat.dtype], sx, sy, sz, connectivity,
+                                          out.ctypes.data, out.itemsize)
+    if rc != 0:
+      raise RuntimeError("reference voxel_connectivity_graph failed")
+    graph = out.reshape((sx, sy, sz), order="F")
+  else:
+    if connectivity in (4, 8):
+      if sz != 1:
+        raise RuntimeError("sz must be 1 for 2D connectivities.")
+      g = _vcg_bits(lab[:, :, 0], VCG_DIRS_2D)[:, :, np.newaxis]
+      mask = 0xFF if connectivity == 8 else 0x0F
+    else:
+      g = _vcg_bits(lab, VCG_DIRS_3D)
+      mask = {26: 0x3FFFFFF, 18: 0x3FFFF, 6: 0x3F}[connectivity]
+    graph = np.asfortranarray((g & np.uint64(mask)).astype(out_dtype))
+  if dims == 3:
+    return graph.reshape((sx, sy, sz), order="F")
+  elif dims == 2:
+    return graph.reshape((sx, sy), order="F")
+  return graph.reshape((sx,), order="F")

@@ -17,6 +17,7 @@
 #include <stdexcept>
 
 #include "cc3d_continuous.hpp"
+#include "cc3d_graphs.hpp"
 
 namespace {
 
@@ -110,6 +111,33 @@ uint64_t ref_estimate_provisional_label_count(const void* in, int in_dtype,
   *first_row = f;
   *last_row = l;
   return r;
+}
+
+// cc3d/cc3d_graphs.hpp:192-236 through cc3d/fastcc3d.pyx:1100-1166: the
+// labels are one of u8/u16/u32/u64 (x fastest), out is zeroed uint8 (out_size
+// 1) or uint32 (4) voxels.  Returns 0, or 1 on a C++ exception.
+int ref_voxel_connectivity_graph(const void* in, int in_dtype, int64_t sx, int64_t sy, int64_t sz,
+                                 int64_t connectivity, void* out, int out_size) {
+  try {
+    auto go = [&](auto* lab) {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(lab)>>;
+      T* l = const_cast<T*>(lab);
+      if (out_size == 1)
+        cc3d::extract_voxel_connectivity_graph<T, uint8_t>(l, sx, sy, sz, connectivity, static_cast<uint8_t*>(out));
+      else
+        cc3d::extract_voxel_connectivity_graph<T, uint32_t>(l, sx, sy, sz, connectivity, static_cast<uint32_t*>(out));
+    };
+    switch (in_dtype) {
+      case DT_U8: go(static_cast<const uint8_t*>(in)); break;
+      case DT_U16: go(static_cast<const uint16_t*>(in)); break;
+      case DT_U32: go(static_cast<const uint32_t*>(in)); break;
+      case DT_U64: go(static_cast<const uint64_t*>(in)); break;
+      default: return 1;
+    }
+  } catch (const std::exception&) {
+    return 1;
+  }
+  return 0;
 }
 
 }  // extern "C"

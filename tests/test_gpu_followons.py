@@ -236,3 +236,50 @@ def test_largest_k_variants(gpu):
       assert _same_array(got[0], exp[0]) and int(got[1]) == int(exp[1]), i
     else:
       assert _same_array(got, exp), i
+
+
+def test_vcg_reference_known_answers(gpu):
+  """automated_test.py:878-950 through the product."""
+  import cc3d
+  labels = np.ones((3, 3), dtype=np.uint8)
+  assert np.all(cc3d.voxel_connectivity_graph(labels, connectivity=8))
+  labels[1, 1] = 0
+  g = cc3d.voxel_connectivity_graph(labels, connectivity=4)
+  gt = np.array([[0x0f, 0b00001011, 0x0f], [0b00001110, 0x00, 0b00001101], [0x0f, 0b00000111, 0x0f]],
+                dtype=np.uint8)
+  assert g.dtype == np.uint8 and np.all(gt.T == g)
+  labels = np.ones((3, 3, 3), dtype=np.uint8)
+  labels[1, 1, 1] = 0
+  g = cc3d.voxel_connectivity_graph(labels, connectivity=26)
+  assert g.dtype == np.uint32 and np.array_equal(g, oracle.voxel_connectivity_graph(labels, connectivity=26))
+  assert int(g[0, 0, 0]) == 0b11111110111111111111111111  # the centre is at (+1, +1, +1)
+
+
+@pytest.mark.parametrize("shape,conns", [((70, 45, 33), (6, 18, 26)), ((300, 257), (4, 8, 6, 26)),
+                                         ((1000,), (26,)), ((1, 40, 9), (18,))])
+@pytest.mark.parametrize("dt", [np.uint8, np.int16, np.uint32, np.uint64, bool])
+def test_vcg_matches_oracle_and_reference(gpu, shape, conns, dt):
+  import cc3d
+  rng = np.random.default_rng(sum(shape))
+  a = rng.integers(0, 3, size=shape)
+  a = (a > 0) if dt == bool else a.astype(dt)
+  for arr in (a, np.asfortranarray(a)):
+    for conn in conns:
+      print("vcg case", shape, np.dtype(dt).name, conn, arr.flags.f_contiguous, flush=True)
+      got = cc3d.voxel_connectivity_graph(arr, connectivity=conn)
+      exp = oracle.voxel_connectivity_graph(arr, connectivity=conn)
+      assert got.dtype == exp.dtype and got.shape == exp.shape and got.flags.f_contiguous
+      assert np.array_equal(got, exp)
+      if oracle.have_ref():
+        assert np.array_equal(got, oracle.voxel_connectivity_graph(arr, connectivity=conn, backend="ref"))
+
+
+def test_vcg_errors(gpu):
+  import cc3d
+  with pytest.raises(ValueError):
+    cc3d.voxel_connectivity_graph(np.zeros((4, 4, 4), np.uint8), connectivity=8)
+  with pytest.raises(TypeError):
+    cc3d.voxel_connectivity_graph(np.zeros((4, 4), np.float32), connectivity=8)
+  with pytest.raises(cc3d.DimensionError):
+    cc3d.voxel_connectivity_graph(np.zeros((2, 2, 2, 2), np.uint8))
+  assert cc3d.voxel_connectivity_graph(np.zeros((0, 3), np.uint8), connectivity=4).shape == (0,)

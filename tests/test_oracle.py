@@ -240,3 +240,66 @@ def test_known_out_dtypes():
     oracle.connected_components(np.arange(41 ** 3, dtype=np.uint32) + 1, out_dtype=np.uint16)
   with pytest.raises(ValueError):
     oracle.connected_components(np.ones((4, 4, 4), np.uint8), out_dtype=np.uint8)
+
+
+def test_vcg_reference_known_answers():
+  """automated_test.py:878-950: the expected bitfields of the reference's own
+  voxel-graph tests (2D 3x3 at 4/8, 3D 3x3x3 at 26, centre voxel zeroed)."""
+  labels = np.ones((3, 3), dtype=np.uint8)
+  assert np.all(oracle.voxel_connectivity_graph(labels, connectivity=4))
+  labels[1, 1] = 0
+  g4 = oracle.voxel_connectivity_graph(labels, connectivity=4)
+  gt4 = np.array([[0x0f, 0b00001011, 0x0f], [0b00001110, 0x00, 0b00001101], [0x0f, 0b00000111, 0x0f]],
+                 dtype=np.uint8)
+  assert g4.dtype == np.uint8 and np.all(gt4.T == g4)
+  g8 = oracle.voxel_connectivity_graph(labels, connectivity=8)
+  gt8 = np.array([[0b11101111, 0b11111011, 0b11011111], [0b11111110, 0x00, 0b11111101],
+                  [0b10111111, 0b11110111, 0b01111111]], dtype=np.uint8)
+  assert np.all(gt8.T == g8)
+  E = {
+    (-1, -1, -1): 0b01111111111111111111111111, (+1, -1, -1): 0b10111111111111111111111111,
+    (-1, +1, -1): 0b11011111111111111111111111, (+1, +1, -1): 0b11101111111111111111111111,
+    (-1, -1, +1): 0b11110111111111111111111111, (+1, -1, +1): 0b11111011111111111111111111,
+    (-1, +1, +1): 0b11111101111111111111111111, (+1, +1, +1): 0b11111110111111111111111111,
+    (0, -1, -1): 0b11111111011111111111111111, (0, +1, -1): 0b11111111101111111111111111,
+    (-1, 0, -1): 0b11111111110111111111111111, (+1, 0, -1): 0b11111111111011111111111111,
+    (0, -1, +1): 0b11111111111101111111111111, (0, +1, +1): 0b11111111111110111111111111,
+    (-1, 0, +1): 0b11111111111111011111111111, (+1, 0, +1): 0b11111111111111101111111111,
+    (-1, -1, 0): 0b11111111111111110111111111, (+1, -1, 0): 0b11111111111111111011111111,
+    (-1, +1, 0): 0b11111111111111111101111111, (+1, +1, 0): 0b11111111111111111110111111,
+    (0, 0, -1): 0b11111111111111111111011111, (0, 0, +1): 0b11111111111111111111101111,
+    (0, -1, 0): 0b11111111111111111111110111, (0, +1, 0): 0b11111111111111111111111011,
+    (-1, 0, 0): 0b11111111111111111111111101, (+1, 0, 0): 0b11111111111111111111111110,
+  }
+  labels = np.ones((3, 3, 3), dtype=np.uint8)
+  g = oracle.voxel_connectivity_graph(labels, connectivity=26)
+  assert g.dtype == np.uint32 and np.all(g == 0x3FFFFFF)
+  labels[1, 1, 1] = 0
+  g = oracle.voxel_connectivity_graph(labels, connectivity=26)
+  # voxel (x, y, z) sees the zeroed centre in direction (1 - x, 1 - y, 1 - z)
+  for x in range(3):
+    for y in range(3):
+      for z in range(3):
+        want = 0 if (x, y, z) == (1, 1, 1) else E[(1 - x, 1 - y, 1 - z)]
+        assert int(g[x, y, z]) == want, (x, y, z)
+
+
+def test_vcg_oracle_matches_reference_build():
+  """The restatement against the reference's extract_voxel_connectivity_graph
+  compiled from its own sources, on random labels of every supported kind."""
+  if not oracle.have_ref():
+    pytest.skip("reference build absent")
+  rng = np.random.default_rng(1021)
+  cases = []
+  for shape in [(17, 9, 5), (1, 7, 3), (6, 1, 1), (33, 20)]:
+    for conn in ([4, 8, 6, 18, 26] if len(shape) == 2 else [6, 18, 26]):
+      for dt in (np.uint8, np.int16, np.uint32, np.int64, bool):
+        cases.append((shape, conn, dt))
+  cases.append(((40,), 26, np.uint16))
+  for shape, conn, dt in cases:
+    a = rng.integers(0, 3, size=shape)
+    a = (a > 0) if dt == bool else a.astype(dt)
+    for arr in (a, np.asfortranarray(a)):
+      exp = oracle.voxel_connectivity_graph(arr, connectivity=conn, backend="ref")
+      got = oracle.voxel_connectivity_graph(arr, connectivity=conn)
+      assert got.dtype == exp.dtype and got.shape == exp.shape and np.array_equal(got, exp), (shape, conn, dt)
