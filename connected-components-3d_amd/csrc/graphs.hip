@@ -12,8 +12,8 @@
 //       bits 0-17, 6-connectivity bits 0-5 (uint8).
 //   2D (4/8): +x -x +y -y (0-3), (+1,+1) (-1,+1) (+1,-1) (-1,-1) (4-7);
 //       4-connectivity keeps bits 0-3.
-// One thread per voxel; the neighbours of a row come through L1/L2, so HBM
-// sees one read of the labels and one write of the graph.
+// A tile and its halo are staged in LDS, so HBM sees about one read of the
+// labels (plus the halo) and one write of the graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,43 +24,76 @@
 
 namespace cc3d_amd {
 
-struct Dir {
-  int8_t dx, dy, dz;
-};
-
-__constant__ Dir g_dir3[26] = {
+constexpr int8_t kDir3[26][3] = {
   {+1, 0, 0},   {-1, 0, 0},   {0, +1, 0},   {0, -1, 0},   {0, 0, +1},   {0, 0, -1},   {+1, +1, 0},
   {-1, +1, 0},  {+1, -1, 0},  {-1, -1, 0},  {+1, 0, +1},  {-1, 0, +1},  {0, +1, +1},  {0, -1, +1},
   {+1, 0, -1},  {-1, 0, -1},  {0, +1, -1},  {0, -1, -1},  {+1, +1, +1}, {-1, +1, +1}, {+1, -1, +1},
   {-1, -1, +1}, {+1, +1, -1}, {-1, +1, -1}, {+1, -1, -1}, {-1, -1, -1},
 };
-__constant__ Dir g_dir2[8] = {
-  {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {+1, +1, 0}, {-1, +1, 0}, {+1, -1, 0}, {-1, -1, 0},
+constexpr int8_t kDir2[8][2] = {
+  {+1, 0}, {-1, 0}, {0, +1}, {0, -1}, {+1, +1}, {-1, +1}, {+1, -1}, {-1, -1},
 };
 
-// One workgroup covers WG consecutive x of one row (y, z); rows are
-// enumerated in a grid-stride loop over blocks.
+// One workgroup per tile of VX x VY x VZ voxels (VZ = 1 in 2D), staged with
+// a one-voxel halo in LDS by coalesced loads; thread (lx, ly) then builds the
+// bitfields of its column of VZ voxels from LDS.  Whether a neighbour exists
+// comes from the voxel's coordinates, not from the staged value.  Tiles are
+// spread over the XCDs in contiguous ranges (xcd_tile) when the grid holds one
+// workgroup per tile, else walked in a grid-stride loop.
+constexpr int VX = 64, VY = 4;
+
+template <bool D2>
+struct VTile {
+  static constexpr int VZ = D2 ? 1 : 4;
+  static constexpr int BX = VX + 2, BY = VY + 2, BZ = D2 ? 1 : VZ + 2;
+  static constexpr int BOX = BX * BY * BZ;
+  static constexpr int Z0 = D2 ? 0 : 1;  // box z of the tile's first plane
+  __host__ __device__ static constexpr int box(int x, int y, int z) { return x + BX * (y + BY * z); }
+};
+
 template <class T, class O, bool D2>
 __global__ __launch_bounds__(WG) void k_vcg(const T* lab, int64_t sx, int64_t sy, int64_t sz, uint32_t keep,
                                             O* out) {
+  using V = VTile<D2>;
   constexpr int ND = D2 ? 8 : 26;
-  const int64_t nxb = (sx + WG - 1) / WG, nblocks = nxb * sy * sz;
-  for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
-    const int64_t row = blk / nxb, x = (blk - row * nxb) * WG + threadIdx.x;
-    if (x >= sx) continue;
-    const int64_t y = row % sy, z = row / sy;
-    const int64_t i = x + sx * row;
-    const T c = lab[i];
-    uint32_t g = D2 ? 0xFFu : 0x3FFFFFFu;
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-      if (!((keep >> d) & 1u)) continue;
-      const Dir o = D2 ? g_dir2[d] : g_dir3[d];
-      const int64_t nx = x + o.dx, ny = y + o.dy, nz = z + o.dz;
-      if (nx < 0 || nx >= sx || ny < 0 || ny >= sy || nz < 0 || nz >= sz) continue;
-      if (lab[i + o.dx + sx * (o.dy + sy * (int64_t)o.dz)] != c) g &= ~(1u << d);
+  __shared__ T s[V::BOX];
+  const int t = threadIdx.x, lx = t % VX, ly = t / VX;
+  const int64_t ntx = (sx + VX - 1) / VX, nty = (sy + VY - 1) / VY, ntz = (sz + V::VZ - 1) / V::VZ;
+  const int64_t ntiles = ntx * nty * ntz;
+  const bool one = (int64_t)gridDim.x == ntiles;
+  for (int64_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const int64_t tile = one ? xcd_tile(b, ntiles) : b;
+    const int64_t tx = tile % ntx, tr = tile / ntx, ty = tr % nty, tz = tr / nty;
+    const int64_t x0 = tx * VX, y0 = ty * VY, z0 = tz * V::VZ;
+    __syncthreads();  // the previous tile's reads are done
+    for (int h = t; h < V::BOX; h += WG) {
+      const int hx = h % V::BX, hy = (h / V::BX) % V::BY, hz = h / (V::BX * V::BY);
+      const int64_t gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - V::Z0;
+      const bool in = gx >= 0 && gx < sx && gy >= 0 && gy < sy && gz >= 0 && gz < sz;
+      s[h] = in ? lab[gx + sx * (gy + sy * gz)] : T(0);
     }
-    out[i] = (O)(g & keep);
+    __syncthreads();
+    const int64_t gx = x0 + lx, gy = y0 + ly;
+    if (gx >= sx || gy >= sy) continue;
+    const bool xm = gx > 0, xp = gx < sx - 1, ym = gy > 0, yp = gy < sy - 1;
+#pragma unroll
+    for (int lz = 0; lz < V::VZ; lz++) {
+      const int64_t gz = z0 + lz;
+      if (gz >= sz) break;
+      const bool zm = gz > 0, zp = gz < sz - 1;
+      const int c0 = V::box(lx + 1, ly + 1, lz + V::Z0);
+      const T c = s[c0];
+      uint32_t g = D2 ? 0xFFu : 0x3FFFFFFu;
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        const int dx = D2 ? kDir2[d][0] : kDir3[d][0], dy = D2 ? kDir2[d][1] : kDir3[d][1];
+        const int dz = D2 ? 0 : kDir3[d][2];
+        const bool ex = (dx < 0 ? xm : dx > 0 ? xp : true) && (dy < 0 ? ym : dy > 0 ? yp : true) &&
+                        (dz < 0 ? zm : dz > 0 ? zp : true);
+        if (ex && s[c0 + dx + V::BX * (dy + V::BY * dz)] != c) g &= ~(1u << d);
+      }
+      out[gx + sx * (gy + sy * gz)] = (O)(g & keep);
+    }
   }
 }
 
@@ -74,8 +107,10 @@ static int gfail(int code, const char* msg) {
 template <class T>
 static void launch_vcg(const void* labels, int64_t sx, int64_t sy, int64_t sz, int connectivity, void* out,
                        hipStream_t s) {
-  const int64_t nblocks = (sx + WG - 1) / WG * sy * sz;
-  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(nblocks, 1 << 20)));
+  const bool d2 = connectivity == 8 || connectivity == 4;
+  const int64_t vz = d2 ? VTile<true>::VZ : VTile<false>::VZ;
+  const int64_t ntiles = (sx + VX - 1) / VX * ((sy + VY - 1) / VY) * ((sz + vz - 1) / vz);
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, 1 << 20)));
   const T* in = static_cast<const T*>(labels);
   switch (connectivity) {
     case 26: hipLaunchKernelGGL((k_vcg<T, uint32_t, false>), grid, dim3(WG), 0, s, in, sx, sy, sz, 0x3FFFFFFu, (uint32_t*)out); break;
