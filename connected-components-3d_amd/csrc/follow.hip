@@ -514,6 +514,9 @@ int cc3d_amd_keep_labels(const void* labels, int label_dtype, int64_t voxels, co
   if (!labels || !out || !lut_scratch || (k > 0 && !keep) || k < 0 || voxels < 0)
     return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: bad argument");
   if (k > 0xFFFFFFFFll) return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: k must fit 32 bits");
+  const auto unsigned_code = [](int d) { return d == DT_U8 || d == DT_U16 || d == DT_U32 || d == DT_U64; };
+  if (!unsigned_code(label_dtype)) return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: labels must be unsigned integers");
+  if (!unsigned_code(out_dtype)) return ffail(CC3D_AMD_ERR_INVALID, "keep_labels: output must be u8/u16/u32/u64");
   const int64_t n1 = (int64_t)N + 1;
   hipLaunchKernelGGL(k_lut_clear, dim3(fgrid(n1, WG, 8192)), dim3(WG), 0, s, lut_scratch, n1);
   if (k > 0) hipLaunchKernelGGL(k_lut_set, dim3(fgrid(k, WG, 8192)), dim3(WG), 0, s, lut_scratch, keep, k, N);
