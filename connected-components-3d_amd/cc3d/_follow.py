@@ -23,6 +23,9 @@ def _bind():
     lib.cc3d_amd_dust_apply.restype = ci
     lib.cc3d_amd_dust_apply.argtypes = [vp, ci, vp, ci, i64, vp, u64, ctypes.c_double, ctypes.c_double,
                                         ci, ci, vp, vp, vp]
+    lib.cc3d_amd_dust_copy.restype = ci
+    lib.cc3d_amd_dust_copy.argtypes = [vp, vp, ci, vp, ci, i64, vp, u64, ctypes.c_double, ctypes.c_double,
+                                       ci, ci, vp, vp, vp]
     lib.cc3d_amd_keep_labels.restype = ci
     lib.cc3d_amd_keep_labels.argtypes = [vp, ci, i64, vp, i64, u64, vp, vp, ci, vp]
     lib.cc3d_amd_follow_last_error.restype = ctypes.c_char_p

@@ -84,15 +84,16 @@ def _followons(vol, labels, N: int, dust_threshold: int):
   V = sx * sy * sz
   lab = labels.as_strided((V,), (1,))
   counts, bbox, _ = _device_stats(lab, L.DT_U32, sx, sy, sz, 3, 1, N, centroids=False)
-  img = vol.detach().clone(memory_format=torch.preserve_format)
+  img = torch.empty_like(vol, memory_format=torch.preserve_format)  # the dusted copy, written in one pass
   flat = img.as_strided((V,), (1,))
+  src = vol.detach().as_strided((V,), (1,))
   erase = torch.empty(N + 1, dtype=torch.uint8, device=vol.device)
   nmask = torch.zeros(1, dtype=torch.int64, device=vol.device)
   code = _NP_TO_CODE[np.dtype({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[vol.element_size()])]
   stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-  _fcheck(_bind().cc3d_amd_dust_apply(
-    ctypes.c_void_p(flat.data_ptr()), code, ctypes.c_void_p(lab.data_ptr()), L.DT_U32, V,
-    ctypes.c_void_p(counts.data_ptr()), N, float(dust_threshold), 0.0, 0, 0,
+  _fcheck(_bind().cc3d_amd_dust_copy(
+    ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(flat.data_ptr()), code, ctypes.c_void_p(lab.data_ptr()),
+    L.DT_U32, V, ctypes.c_void_p(counts.data_ptr()), N, float(dust_threshold), 0.0, 0, 0,
     ctypes.c_void_p(erase.data_ptr()), ctypes.c_void_p(nmask.data_ptr()), stream))
   return {"labels": labels, "N": N, "voxel_counts": counts, "bounding_boxes": bbox.view(N + 1, 6),
           "dusted": img, "_nmask": nmask}

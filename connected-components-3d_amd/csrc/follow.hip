@@ -329,6 +329,13 @@ __global__ __launch_bounds__(WG) void k_keep_apply(O* out, const L* lab, const u
   }
 }
 
+template <class T, class L>
+__global__ __launch_bounds__(WG) void k_dust_copy(T* __restrict__ dst, const T* __restrict__ src, const L* lab,
+                                                 const uint8_t* erase, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (int64_t)gridDim.x * WG)
+    dst[i] = erase[(uint64_t)lab[i]] ? T(0) : src[i];
+}
+
 static thread_local std::string f_err;
 
 static int ffail(int code, const char* msg) {
@@ -542,6 +549,43 @@ int cc3d_amd_keep_labels(const void* labels, int label_dtype, int64_t voxels, co
 #undef CC3D_KO
 #undef CC3D_KA
   if (hipGetLastError() != hipSuccess) return ffail(CC3D_AMD_ERR_HIP, "keep_labels launch failed");
+  return CC3D_AMD_OK;
+}
+
+// dust into a separate image: dst[v] = 0 where the component of v fails the
+// test of cc3d_amd_dust_apply, else src[v] (the copy and the masking of
+// cc3d/__init__.py:121-150 in one pass).  Same arguments otherwise.
+int cc3d_amd_dust_copy(const void* src, void* dst, int img_dtype, const void* labels, int label_dtype,
+                       int64_t voxels, const uint32_t* counts, uint64_t N, double lo, double hi, int ranged,
+                       int invert, uint8_t* erase_scratch, unsigned long long* n_masked, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!src || !dst || !labels || !counts || !erase_scratch || !n_masked || voxels < 0)
+    return ffail(CC3D_AMD_ERR_INVALID, "dust_copy: bad argument");
+  const auto unsigned_code = [](int d) { return d == DT_U8 || d == DT_U16 || d == DT_U32 || d == DT_U64; };
+  if (!unsigned_code(label_dtype)) return ffail(CC3D_AMD_ERR_INVALID, "labels must be unsigned integers");
+  if (!unsigned_code(img_dtype) && img_dtype != DT_F32 && img_dtype != DT_F64)
+    return ffail(CC3D_AMD_ERR_INVALID, "bad image dtype");
+  hipLaunchKernelGGL(k_dust_table, dim3(fgrid((int64_t)N + 1, WG, 8192)), dim3(WG), 0, s, counts, (int64_t)N, lo,
+                     hi, ranged, invert, erase_scratch, n_masked);
+  const dim3 g(fgrid(voxels, WG, 65536));
+#define CC3D_DC(T, L) \
+  hipLaunchKernelGGL((k_dust_copy<T, L>), g, dim3(WG), 0, s, (T*)dst, (const T*)src, (const L*)labels, erase_scratch, voxels)
+#define CC3D_DCL(T)                                 \
+  switch (label_dtype) {                            \
+    case DT_U8: CC3D_DC(T, uint8_t); break;         \
+    case DT_U16: CC3D_DC(T, uint16_t); break;       \
+    case DT_U32: CC3D_DC(T, uint32_t); break;       \
+    default: CC3D_DC(T, uint64_t); break;           \
+  }
+  switch (img_dtype) {  // zeroing is bitwise: +0.0 has all bits clear
+    case DT_U8: CC3D_DCL(uint8_t); break;
+    case DT_U16: CC3D_DCL(uint16_t); break;
+    case DT_U32: case DT_F32: CC3D_DCL(uint32_t); break;
+    default: CC3D_DCL(uint64_t); break;
+  }
+#undef CC3D_DCL
+#undef CC3D_DC
+  if (hipGetLastError() != hipSuccess) return ffail(CC3D_AMD_ERR_HIP, "dust launch failed");
   return CC3D_AMD_OK;
 }
 

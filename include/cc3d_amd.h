@@ -169,6 +169,15 @@ int cc3d_amd_dust_apply(void* img, int img_dtype, const void* labels, int label_
                         int64_t voxels, const uint32_t* counts, uint64_t N, double lo, double hi,
                         int ranged, int invert, uint8_t* erase_scratch,
                         unsigned long long* n_masked, void* stream);
+/* cc3d_amd_dust_apply into a separate image: dst = src with the failing
+ * components zeroed, the copy and the masking of cc3d/__init__.py:121-150 in
+ * one pass (dst and src must not overlap).  For callers that keep the
+ * original image; the arguments otherwise mean the same. */
+int cc3d_amd_dust_copy(const void* src, void* dst, int img_dtype, const void* labels, int label_dtype,
+                       int64_t voxels, const uint32_t* counts, uint64_t N, double lo, double hi,
+                       int ranged, int invert, uint8_t* erase_scratch,
+                       unsigned long long* n_masked, void* stream);
+
 /* The renumbering step of cc3d.largest_k without fastremap
  * (cc3d/__init__.py:250-275): out[v] = i + 1 where labels[v] == keep[i], else
  * 0 (uint8 output with k == 1: the boolean mask).  keep: device uint64[k], the
