@@ -44,7 +44,7 @@ constexpr int VX = 64, VY = 4;
 
 template <bool D2>
 struct VTile {
-  static constexpr int VZ = D2 ? 1 : 4;
+  static constexpr int VZ = D2 ? 1 : 8;
   static constexpr int BX = VX + 2, BY = VY + 2, BZ = D2 ? 1 : VZ + 2;
   static constexpr int BOX = BX * BY * BZ;
   static constexpr int Z0 = D2 ? 0 : 1;  // box z of the tile's first plane
