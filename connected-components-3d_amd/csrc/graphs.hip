@@ -12,8 +12,8 @@
 //       bits 0-17, 6-connectivity bits 0-5 (uint8).
 //   2D (4/8): +x -x +y -y (0-3), (+1,+1) (-1,+1) (+1,-1) (-1,-1) (4-7);
 //       4-connectivity keeps bits 0-3.
-// A tile and its halo are staged in LDS, so HBM sees about one read of the
-// labels (plus the halo) and one write of the graph.
+// A tile and its halo are staged in LDS; the halo re-reads partly hit L2, so
+// HBM sees the labels read 1.1 to 1.5 times and the graph written once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
