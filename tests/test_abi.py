@@ -1,7 +1,8 @@
 """The C-ABI library loads and exports every entry point include/cc3d_amd.h declares.
 
-CPU-only: loading the library and resolving symbols needs no GPU; no compute
-entry point is called here.
+CPU-only: loading the library and resolving symbols needs no GPU.  Entry
+points are called here only with arguments they reject before any device
+work (argument checks return a status code and a message).
 """
 import ctypes
 import os
@@ -36,3 +37,41 @@ def test_version_string():
   from cc3d import _lib
   v = _lib.lib().cc3d_amd_version().decode()
   assert "gfx950" in v
+
+
+def test_argument_checks_without_device_work():
+  from cc3d import _lib
+  lib = _lib.lib()
+  vp, i64, u64, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int
+  buf = ctypes.create_string_buffer(64)  # never read: every call below is rejected first
+  p = ctypes.cast(buf, vp)
+
+  f = lib.cc3d_amd_voxel_connectivity_graph
+  f.restype, f.argtypes = ci, [vp, ci, i64, i64, i64, ci, vp, vp]
+  lib.cc3d_amd_graphs_last_error.restype = ctypes.c_char_p
+  assert f(p, 0, 4, 4, 4, 5, p, None) == _lib.ERR_RUNTIME
+  assert b"Only 4 and 8 2D" in lib.cc3d_amd_graphs_last_error()
+  assert f(p, 0, 4, 4, 2, 8, p, None) == _lib.ERR_RUNTIME
+  assert lib.cc3d_amd_graphs_last_error() == b"sz must be 1 for 2D connectivities."
+  assert f(p, 0, 0, 4, 4, 26, p, None) == _lib.ERR_INVALID
+  assert f(p, 4, 4, 4, 4, 26, p, None) == _lib.ERR_INVALID  # float labels
+
+  lib.cc3d_amd_follow_last_error.restype = ctypes.c_char_p
+  k = lib.cc3d_amd_keep_labels
+  k.restype, k.argtypes = ci, [vp, ci, i64, vp, i64, u64, vp, vp, ci, vp]
+  assert k(p, 4, 8, p, 1, 3, p, p, 2, None) == _lib.ERR_INVALID  # float labels
+  assert k(p, 2, 8, p, 1, 3, p, p, 5, None) == _lib.ERR_INVALID  # float output
+  assert k(p, 2, 8, p, -1, 3, p, p, 2, None) == _lib.ERR_INVALID
+  assert b"keep_labels" in lib.cc3d_amd_follow_last_error()
+
+  d = lib.cc3d_amd_dust_copy
+  d.restype = ci
+  d.argtypes = [vp, vp, ci, vp, ci, i64, vp, u64, ctypes.c_double, ctypes.c_double, ci, ci, vp, vp, vp]
+  assert d(p, p, 2, p, 4, 8, p, 3, 1.0, 0.0, 0, 0, p, p, None) == _lib.ERR_INVALID  # float labels
+  assert d(None, p, 2, p, 2, 8, p, 3, 1.0, 0.0, 0, 0, p, p, None) == _lib.ERR_INVALID
+
+  s = lib.cc3d_amd_statistics
+  s.restype, s.argtypes = ci, [vp, ci, i64, i64, i64, ci, ci, u64, vp, vp, vp, vp, vp]
+  assert s(p, 2, 4, 4, 4, 3, 1, 3, None, p, None, None, None) == _lib.ERR_INVALID  # counts missing
+  assert s(p, 2, 4, 4, 4, 3, 1, 3, p, p, p, None, None) == _lib.ERR_INVALID  # centroids need sums
+  assert s(p, 4, 4, 4, 4, 3, 1, 3, p, p, None, None, None) == _lib.ERR_INVALID  # float labels
