@@ -345,7 +345,7 @@ def main() -> int:
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-          "(volumes are independent; see the sharding notes in DESIGN.md)", file=sys.stderr)
+          "(volumes are independent: each GPU would serve its own queue of volumes)", file=sys.stderr)
     return 2
 
   import torch
