@@ -156,9 +156,11 @@ struct GlobalParams {
   uint32_t* bsum;
   int64_t nwords;
   Counters* ctr;
-  // batch of volumes (nvol > 0): per-volume table and first tile of each
+  // batch of volumes (nvol > 0): per-volume table, first tile of each, and
+  // the table entry of each tile
   const BatchVol* vols;
   const int64_t* vtile0;
+  const uint32_t* vix;
   int nvol;
 };
 
@@ -178,7 +180,7 @@ __device__ inline VolGeo batch_geo(const BatchVol& v) {
 }
 
 __device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
-  if (p.nvol != 0) return batch_geo(p.vols[find_vol(p.vtile0, p.nvol, slot)]);
+  if (p.nvol != 0) return batch_geo(p.vols[p.vix[slot]]);
   VolGeo g;
   g.sx = p.sx; g.sy = p.sy; g.sz = p.sz; g.osx = p.osx;
   g.ntx = p.ntx; g.nty = p.nty; g.ntz = p.ntz;
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
   for (int n = 0; n < FIN_TILES; n++) {
     const int64_t slot = first + n;
     if (slot >= p.ntiles) break;
-    vi[n] = find_vol(p.vtile0, p.nvol, slot);
+    vi[n] = (int)p.vix[slot];
     fin_load<G>(p, slot, p.prefix[p.vols[vi[n]].key0 >> 5], j[n], lab[n]);
   }
   __syncthreads();
@@ -572,6 +574,12 @@ __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
     else if (bv.out_dtype == DT_U64) fin_store<uint64_t, G>(v, tile, static_cast<uint64_t*>(bv.out), j[n], lab[n]);
     else fin_store<uint32_t, G>(v, tile, static_cast<uint32_t*>(bv.out), j[n], lab[n]);
   }
+}
+
+// Batch: the table entry of every tile (one block per volume).
+__global__ __launch_bounds__(WG) void k_fill_vix(const int64_t* vtile0, uint32_t* vix) {
+  const int v = blockIdx.x;
+  for (int64_t i = vtile0[v] + threadIdx.x; i < vtile0[v + 1]; i += WG) vix[i] = (uint32_t)v;
 }
 
 // Batch: per-volume counters from the per-tile records (one block per volume).
@@ -1099,7 +1107,7 @@ struct cc3d_amd_batch {
   bool own_ws = false;
   size_t total = 0;
   int64_t ntiles = 0, nkeys = 0, nwords = 0, nblk = 0;
-  size_t oVOL = 0, oT0 = 0, oGP = 0, oRL = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0, oBITS = 0,
+  size_t oVOL = 0, oT0 = 0, oVIX = 0, oGP = 0, oRL = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0, oBITS = 0,
          oPRE = 0, oBSUM = 0, oCTR = 0;
   BatchVol* h_tab = nullptr;  // pinned host copy of the volume table
   int64_t* h_t0 = nullptr;    // pinned: first tile of each volume (n + 1)
@@ -1674,6 +1682,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   const size_t slots = (size_t)tiles * Geo3::TV;
   b.oVOL = take((size_t)n * sizeof(BatchVol));
   b.oT0 = take((size_t)(n + 1) * sizeof(int64_t));
+  b.oVIX = take((size_t)tiles * 4);
   b.oGP = take((size_t)b.nkeys * 4);
   b.oRL = take(slots * 4);
   b.oTR = take((size_t)tiles * 4);
@@ -1765,6 +1774,13 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
   gp.ctr = reinterpret_cast<Counters*>(ws + b->oCTR);
   gp.vols = reinterpret_cast<const BatchVol*>(ws + b->oVOL);
   gp.vtile0 = reinterpret_cast<const int64_t*>(ws + b->oT0);
+  gp.vix = reinterpret_cast<const uint32_t*>(ws + b->oVIX);
+  hipLaunchKernelGGL(k_fill_vix, dim3((unsigned)n), dim3(WG), 0, b->stream, gp.vtile0,
+                     reinterpret_cast<uint32_t*>(ws + b->oVIX));
+  if (hipGetLastError() != hipSuccess) {
+    cc3d_amd_batch_free(b);
+    return fail(CC3D_AMD_ERR_HIP, "could not index the batch tiles");
+  }
   gp.nvol = n;
   LocalParams& lp = b->lp;
   rc = utab_ready(&lp.utab);
@@ -1820,9 +1836,9 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     ProfScope ps_("k_local", s);
     for (const auto& g : b->groups) {
       if (b->mode == M_BINARY)
-        e = launch_local_batch_binary(b->dt, g.nb, b->lp, p.vols, p.vtile0, b->n, g.tile0, g.tiles, s);
+        e = launch_local_batch_binary(b->dt, g.nb, b->lp, p.vols, p.vix, g.tile0, g.tiles, s);
       else
-        e = launch_local_batch_multi(b->dt, g.nb, b->lp, p.vols, p.vtile0, b->n, g.tile0, g.tiles, s);
+        e = launch_local_batch_multi(b->dt, g.nb, b->lp, p.vols, p.vix, g.tile0, g.tiles, s);
       if (e != hipSuccess) break;
     }
   }

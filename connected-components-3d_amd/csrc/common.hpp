@@ -55,17 +55,6 @@ struct BatchVol {
   int nb, out_dtype;
 };
 
-// Volume of a batch owning batch tile `slot` (vtile0: first tile of each volume).
-__device__ inline int find_vol(const int64_t* vtile0, int n, int64_t slot) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (vtile0[mid] <= slot) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // Tile geometry.  A tile is TX x TY x TZ voxels; the local kernel's
 // workgroup of NT threads (one column of PER_THREAD voxels each) stages it in
 // LDS together with a one-voxel halo on the sides that backward neighbours

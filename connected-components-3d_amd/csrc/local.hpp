@@ -1167,15 +1167,15 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>()
 
 // The volumes of a batch that share a connectivity, in one launch: their
 // tiles are contiguous in the batch, [gtile0, gtile0 + gtiles); each
-// workgroup finds its volume in the table.  p carries the shared per-tile
+// workgroup reads its volume's table entry from vix.  p carries the shared per-tile
 // arrays; the geometry fields come from the table.  Queue capacity and
 // occupancy are those of the single-volume kernel of the same connectivity.
 template <class T, int MODE, int NB, class G>
 __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>())) void k_local_batch(
-  LocalParams p, const BatchVol* vols, const int64_t* vtile0, int nvol, int64_t gtile0, int64_t gtiles) {
+  LocalParams p, const BatchVol* vols, const uint32_t* vix, int64_t gtile0, int64_t gtiles) {
   __shared__ LocalLds<T, G, full_queue<MODE, NB>()> L;
   const int64_t slot = gtile0 + xcd_tile(blockIdx.x, gtiles);
-  const BatchVol& v = vols[find_vol(vtile0, nvol, slot)];
+  const BatchVol& v = vols[vix[slot]];
   LocalParams q = p;
   q.in = v.in;
   q.sx = v.sx;
@@ -1195,17 +1195,17 @@ hipError_t launch_local_binary(int dt, int nb, int qk, const LocalParams& p, hip
 hipError_t launch_local_cont(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
 // A batch launches once per connectivity: tiles [gtile0, gtile0 + gtiles) all
 // belong to volumes of connectivity nb.
-hipError_t launch_local_batch_multi(int dt, int nb, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
-                                    int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s);
-hipError_t launch_local_batch_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols,
-                                     const int64_t* vtile0, int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_multi(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
+                                    int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
+                                     int64_t gtile0, int64_t gtiles, hipStream_t s);
 
 template <int MODE>
-hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols, const int64_t* vtile0,
-                                   int nvol, int64_t gtile0, int64_t gtiles, hipStream_t s) {
+hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
+                                   int64_t gtile0, int64_t gtiles, hipStream_t s) {
   const dim3 grid((unsigned)gtiles), block(Geo3::NT);
 #define CC3D_B(T, NB) \
-  hipLaunchKernelGGL((k_local_batch<T, MODE, NB, Geo3>), grid, block, 0, s, p, vols, vtile0, nvol, gtile0, gtiles)
+  hipLaunchKernelGGL((k_local_batch<T, MODE, NB, Geo3>), grid, block, 0, s, p, vols, vix, gtile0, gtiles)
 #define CC3D_BNB(T)                          \
   switch (nb) {                              \
     case 26: CC3D_B(T, 26); break;           \
