@@ -298,9 +298,17 @@ __global__ void k_dust_table(const uint32_t* counts, int64_t N, double lo, doubl
     m += mask;
     erase[i] = (uint8_t)(mask != (invert != 0));
   }
+  // one atomic per block: the count lands on a single address
+  __shared__ unsigned long long wsum[WG / 64];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(nmask, m);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+    for (int w = 0; w < WG / 64; w++) b += wsum[w];
+    if (b) atomicAdd(nmask, b);
+  }
 }
 
 template <class T, class L>
