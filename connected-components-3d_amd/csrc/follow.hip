@@ -45,7 +45,21 @@ __global__ __launch_bounds__(WG) void k_label_range(const L* lab, int64_t n, int
     lo = b < lo ? b : lo;
     hi = c > hi ? c : hi;
   }
+  // one set of atomics per block: each lands on a single address
+  __shared__ unsigned long long wmx[WG / 64];
+  __shared__ long long wlo[WG / 64], whi[WG / 64];
   if ((threadIdx.x & 63) == 0) {
+    wmx[threadIdx.x >> 6] = mx;
+    wlo[threadIdx.x >> 6] = lo;
+    whi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < WG / 64; w++) {
+      mx = wmx[w] > mx ? wmx[w] : mx;
+      lo = wlo[w] < lo ? wlo[w] : lo;
+      hi = whi[w] > hi ? whi[w] : hi;
+    }
     atomicMax(umax, mx);
     atomicMin(smin, lo);
     atomicMax(smax, hi);
