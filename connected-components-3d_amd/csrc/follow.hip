@@ -132,7 +132,7 @@ template <class L, int W>
 __global__ __launch_bounds__(WG) void k_stats(const L* lab, StatGeo g, uint32_t* counts, uint32_t* bbox,
                                               unsigned long long* sums) {
   constexpr bool BB = (W & 1) != 0, SM = (W & 2) != 0, CF = (W & 4) != 0;
-  constexpr unsigned long long EMPTY = ~0ull;
+  constexpr unsigned long long EMPTY = ~0ull;  // never a label: labels are at most N < 2^64 - 1
   __shared__ unsigned long long tkey[ST_SLOTS];
   __shared__ uint32_t tcnt[ST_SLOTS];
   __shared__ uint32_t tmn[BB ? 3 * ST_SLOTS : 1], tmx[BB ? 3 * ST_SLOTS : 1];
