@@ -20,8 +20,8 @@ from typing import Any, List, Optional, Sequence
 import numpy as np
 
 from . import _lib as L
-from ._core import (_OUT_CODE, _decide, _deliver, _label_one, _prepare, _require_gpu, _stream, _torch,
-                    _torch_out_dtype)
+from ._core import (_OUT_CODE, _decide, _deliver, _label_one, _prepare, _require_gpu, _special_row, _stream,
+                    _torch, _torch_out_dtype)
 
 
 def _batchable(pl) -> bool:
@@ -81,6 +81,12 @@ def _label_group(items, results, errors):
     for k, (i, pl) in enumerate(items):
       if i in errors:
         continue
+      if pl.special_row is not None:
+        try:
+          Ns[k] = _special_row(pl, bufs[k], odts[k], outs[k])
+        except Exception as e:  # the reference raises for this volume
+          errors[i] = e
+          continue
       results[i] = _deliver(outs[k], pl.vol, pl.sx, pl.sy, pl.sz, pl.dims, pl.order, odts[k], None, Ns[k],
                             pl.return_N)
   finally:
@@ -190,6 +196,8 @@ class LabelBatch:
     res = []
     for k, pl in enumerate(self.plans):
       odt, N = _decide(pl, st[k])
+      if pl.special_row is not None:
+        N = _special_row(pl, self.bufs[k], odt, self.outs[k])
       if deliver:
         res.append(_deliver(self.outs[k], pl.vol, pl.sx, pl.sy, pl.sz, pl.dims, pl.order, odt, None, N, True))
       else:

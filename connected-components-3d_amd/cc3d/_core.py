@@ -332,6 +332,7 @@ def _decide(pl, st):
   odt = _choose_out_dtype(pl, max_labels)
 
   special_row = (pl.delta == 0 and ffr == lfr and ffr >= 0)
+  pl.special_row = ffr if special_row else None
   N = int(st.N)
   if not special_row:
     uf_len = int(L.lib().cc3d_amd_union_find_length(
@@ -390,11 +391,53 @@ def _label_one(pl):
     L.check(L.lib().cc3d_amd_job_stats(job, ctypes.byref(st)))
     odt, N = _decide(pl, st)
     out_dev = torch.empty(pl.voxels, dtype=_torch_out_dtype(odt), device=buf.device)
-    L.check(L.lib().cc3d_amd_label_finish(job, ctypes.c_void_p(out_dev.data_ptr()), _OUT_CODE[odt]))
+    if pl.special_row is None:
+      L.check(L.lib().cc3d_amd_label_finish(job, ctypes.c_void_p(out_dev.data_ptr()), _OUT_CODE[odt]))
   finally:
     L.lib().cc3d_amd_job_free(job)
+  if pl.special_row is not None:
+    N = _special_row(pl, buf, odt, out_dev)
 
   return _deliver(out_dev, pl.vol, sx, sy, sz, pl.dims, pl.order, odt, pl.out_file, N, pl.return_N)
+
+
+_ROW_FAILURES = {
+  1: (ValueError, "cannot convert float NaN to integer"),
+  2: (OverflowError, "cannot convert float infinity to integer"),
+  3: (OverflowError, "Python int too large to convert to C long"),
+}
+
+
+def _special_row(pl, buf, odt, out_dev) -> int:
+  """epl_special_row (cc3d/fastcc3d.pyx:644-679) on the GPU: the binding labels
+  a volume whose only foreground is one row (delta == 0) without the C++ core.
+
+  Its `cdef int64_t last_label` compares each voxel with the int64 conversion
+  of its run's first value, so float runs break on non-integer values, and the
+  conversion raises for NaN, infinities and values outside the int64 range
+  (uint64 >= 2^63 included).  Each label is stored into the output before the
+  value is converted, so a label past the output dtype's range raises first
+  (numpy's out-of-bounds OverflowError).  Writes every voxel of out_dev and
+  returns N."""
+  n = ctypes.c_uint64(0)
+  kind = ctypes.c_int(0)
+  rank = ctypes.c_uint64(0)
+  check_u64 = int(pl.vol.np_dtype == np.uint64)
+  lib = L.lib()
+  rc = lib.cc3d_amd_special_row(
+    ctypes.c_void_p(buf.data_ptr()), pl.code, check_u64, pl.sx, pl.voxels, pl.special_row,
+    ctypes.c_void_p(out_dev.data_ptr()), _OUT_CODE[odt], ctypes.byref(n), ctypes.byref(kind),
+    ctypes.byref(rank), _stream())
+  if rc != L.OK:
+    msg = lib.cc3d_amd_special_row_last_error().decode(errors="replace")
+    raise (MemoryError if rc == L.ERR_NOMEM else ValueError if rc == L.ERR_INVALID else RuntimeError)(msg)
+  outmax = int(np.iinfo(odt).max)
+  if kind.value and int(rank.value) <= outmax:
+    exc, msg = _ROW_FAILURES[kind.value]
+    raise exc(msg)
+  if int(n.value) > outmax:
+    raise OverflowError(f"Python integer {outmax + 1} out of bounds for {np.dtype(odt).name}")
+  return int(n.value)
 
 
 def _deliver(out_dev, vol, sx, sy, sz, dims, order, out_dtype, out_file, N, return_N):

@@ -164,14 +164,18 @@ def dust(img, threshold: Union[int, float, tuple, list], connectivity: int = 26,
   _require_gpu()
 
   if precomputed_ccl:
-    N = int(np.max(img)) if img.size else 0
-  vol = _Vol(img)
-  img_dev = vol.device_buffer(order)
-  if precomputed_ccl:
+    # the reference's statistics() checks on the (unsigned view of the)
+    # labels: N = max label must not exceed the voxel count
+    # (cc3d/fastcc3d.pyx:731-737); the labels are then resident in HBM
+    lv = _label_volume(img) if img.size else None
+    N = lv[-1] if lv is not None else 0
+    img_dev = lv[0] if lv is not None else _Vol(img).device_buffer(order)
     cc_dev = img_dev
     cc_dtype = np.dtype(img.dtype) if img.dtype != np.bool_ else np.dtype(np.uint8)
   else:
-    tview = torch.as_strided(img_dev, img.shape, _strides_of(img))
+    img_dev = _Vol(img).device_buffer(order)
+    # img_dev holds img packed in `order` (a non-contiguous view is copied)
+    tview = torch.as_strided(img_dev, img.shape, _packed_strides(img.shape, order))
     if img.dtype == np.bool_:
       tview = tview.view(torch.bool)  # keeps the reference's bool => binary rule
     cc_t, N = connected_components(tview, connectivity=connectivity, return_N=True, binary_image=bool(binary_image))
@@ -207,15 +211,25 @@ def dust(img, threshold: Union[int, float, tuple, list], connectivity: int = 26,
       img = np.zeros(img.shape, dtype=img.dtype, order="F")
     return (img, dust_N) if return_N else img
 
-  host = img_dev.cpu().numpy()
-  flat = np.ravel(img, order="K")
-  flat[:] = host.view(flat.dtype)
+  host = img_dev.cpu().numpy().view(img.dtype if img.dtype != np.bool_ else np.uint8)
+  if img.dtype == np.bool_:
+    host = host.view(np.bool_)
+  # write through the caller's array (a strided view included), as the
+  # reference's img[mask] = 0 does
+  img[...] = host.reshape(img.shape, order=order)
   img = img.view(orig_dtype)
   return (img, dust_N) if return_N else img
 
 
-def _strides_of(a: np.ndarray):
-  return tuple(s // a.itemsize for s in a.strides)
+def _packed_strides(shape, order: str):
+  """Element strides of a dense array of `shape` in memory order `order`."""
+  strides = [0] * len(shape)
+  acc = 1
+  axes = range(len(shape) - 1, -1, -1) if order == "C" else range(len(shape))
+  for a in axes:
+    strides[a] = acc
+    acc *= shape[a]
+  return tuple(strides)
 
 
 def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0, return_N: bool = False,
@@ -245,8 +259,9 @@ def largest_k(img, k: int, connectivity: int = 26, delta: Union[int, float] = 0,
   elif img.size and (img.flags.c_contiguous or img.flags.f_contiguous) and img.dtype != np.float16:
     # the labels stay in HBM: one copy of the image in, one of the result out
     _require_gpu()
-    img_dev = _Vol(img).device_buffer("F" if img.flags.f_contiguous else "C")
-    tview = torch.as_strided(img_dev, img.shape, _strides_of(img))
+    io = "F" if img.flags.f_contiguous else "C"
+    img_dev = _Vol(img).device_buffer(io)
+    tview = torch.as_strided(img_dev, img.shape, _packed_strides(img.shape, io))
     if img.dtype == np.bool_:
       tview = tview.view(torch.bool)  # keeps the reference's bool => binary rule
     cc_t, N = connected_components(tview, connectivity=connectivity, return_N=True, delta=delta,

@@ -90,6 +90,11 @@ def lib():
   L.cc3d_amd_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
   L.cc3d_amd_batch_free.restype = None
   L.cc3d_amd_batch_free.argtypes = [vp]
+  L.cc3d_amd_special_row.restype = ci
+  L.cc3d_amd_special_row.argtypes = [
+    vp, ci, ci, i64, i64, i64, vp, ci, ctypes.POINTER(u64), ctypes.POINTER(ci), ctypes.POINTER(u64), vp]
+  L.cc3d_amd_special_row_last_error.restype = ctypes.c_char_p
+  L.cc3d_amd_special_row_last_error.argtypes = []
   L.cc3d_amd_last_error.restype = ctypes.c_char_p
   L.cc3d_amd_last_error.argtypes = []
   L.cc3d_amd_version.restype = ctypes.c_char_p

@@ -1545,6 +1545,14 @@ int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, in
   const bool continuous = !binary_image && !delta_is_zero(in_dtype, db);
   const int64_t V = sx * sy * sz;
   *N = 0;
+  // The reference dispatcher's own checks come before any first pass
+  // (cc3d/cc3d_continuous.hpp:407-454, cc3d/cc3d.hpp:1432-1470).
+  if (continuous && periodic_boundary)
+    return fail(CC3D_AMD_ERR_RUNTIME, "periodic_boundary is not currently supported for continuous data.");
+  if (connectivity != 26 && connectivity != 18 && connectivity != 6 && connectivity != 8 && connectivity != 4)
+    return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
+  if ((connectivity == 8 || connectivity == 4) && sz != 1)
+    return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
   const uint64_t L = cc3d_amd_union_find_length(max_labels, V, connectivity, binary_image, out_dtype);
   if (L == 0) {
     // the reference returns before its first pass: no foreground is labelled

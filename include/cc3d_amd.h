@@ -53,6 +53,24 @@ int cc3d_amd_estimate_provisional_label_count(const void* in, int in_dtype, int6
                                               int64_t* first_foreground_row,
                                               int64_t* last_foreground_row, void* stream);
 
+/* Replaces epl_special_row (cc3d/fastcc3d.pyx:644-679), the binding's
+ * shortcut for a volume whose only foreground is row `row` (y + sy * z) when
+ * delta == 0 (cc3d/fastcc3d.pyx:469-470).  Writes every voxel of `out`: the
+ * row's labels, 0 elsewhere.  The reference keeps the run value in a C
+ * int64, so a voxel continues a run when it equals the truncation of the
+ * previous voxel; *N receives the number of labels.  *fail_kind reports the
+ * first start voxel whose int64 conversion raises in the reference (0 none,
+ * 1 NaN -> ValueError, 2 infinity -> OverflowError, 3 out of int64 range ->
+ * OverflowError; uint64 inputs are checked when check_u64 != 0) and
+ * *fail_rank that start's label.  The caller raises: the conversion error
+ * when fail_rank fits out_dtype, else numpy's out-of-bounds OverflowError
+ * when N does not fit.  in_dtype as above (signed inputs as their unsigned
+ * view). */
+int cc3d_amd_special_row(const void* in, int in_dtype, int check_u64, int64_t sx, int64_t voxels, int64_t row,
+                         void* out, int out_dtype, uint64_t* N, int* fail_kind, uint64_t* fail_rank,
+                         void* stream);
+const char* cc3d_amd_special_row_last_error(void);
+
 /* ---- two-stage form used by the Python binding ----------------------------
  * The binding layer (cc3d/fastcc3d.pyx:244-626) needs the provisional-label
  * estimate before it picks the output dtype.  Stage 1 labels the volume and

@@ -544,3 +544,56 @@ int oracle_label(const void* in, int dt, int64_t sx, int64_t sy, int64_t sz,
   free(rank);
   return 0;
 }
+
+/*
+ * oracle_statistics: cc3d.statistics restated (cc3d/fastcc3d.pyx:681-765,
+ * _statistics_helper2d :771-844, _statistics_helper3d :850-938).  labels is
+ * the unsigned view (dt 0-3) of an array of numpy shape (s0, s1, s2) (s2 = 1
+ * for 2D) in F (f_order = 1, axis 0 fastest) or C order.  Per label 0..N:
+ * counts (uint32, wrapping like the reference's uint32 array), per numpy
+ * axis the min and max coordinate (bbox[l*6 + 2a], bbox[l*6 + 2a + 1]; mins
+ * start at UINT32_MAX and maxes at 0 for absent labels) and the sum of
+ * coordinates (sums[l*3 + a]; the reference accumulates them in float64,
+ * where they are exact integers below 2^53).  Returns 1 if a label exceeds
+ * N.
+ */
+int oracle_statistics(const void* labels, int dt, int64_t s0, int64_t s1, int64_t s2, int f_order,
+                      uint64_t N, uint32_t* counts, uint32_t* bbox, uint64_t* sums) {
+  for (uint64_t l = 0; l <= N; l++) {
+    counts[l] = 0;
+    for (int a = 0; a < 3; a++) {
+      bbox[l * 6 + 2 * a] = 0xFFFFFFFFu;
+      bbox[l * 6 + 2 * a + 1] = 0;
+      sums[l * 3 + a] = 0;
+    }
+  }
+  const int64_t V = s0 * s1 * s2;
+  for (int64_t i = 0; i < V; i++) {
+    uint64_t l;
+    switch (dt) {
+      case DT_U8: l = ((const uint8_t*)labels)[i]; break;
+      case DT_U16: l = ((const uint16_t*)labels)[i]; break;
+      case DT_U32: l = ((const uint32_t*)labels)[i]; break;
+      default: l = ((const uint64_t*)labels)[i]; break;
+    }
+    if (l > N) return 1;
+    int64_t c[3];
+    if (f_order) {
+      c[0] = i % s0;
+      c[1] = (i / s0) % s1;
+      c[2] = i / (s0 * s1);
+    } else {
+      c[2] = i % s2;
+      c[1] = (i / s2) % s1;
+      c[0] = i / (s2 * s1);
+    }
+    counts[l] += 1u;
+    for (int a = 0; a < 3; a++) {
+      const uint32_t v = (uint32_t)c[a];
+      if (v < bbox[l * 6 + 2 * a]) bbox[l * 6 + 2 * a] = v;
+      if (v > bbox[l * 6 + 2 * a + 1]) bbox[l * 6 + 2 * a + 1] = v;
+      sums[l * 3 + a] += (uint64_t)c[a];
+    }
+  }
+  return 0;
+}

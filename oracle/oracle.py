@@ -32,6 +32,8 @@ MODE_MULTI, MODE_BINARY, MODE_CONT = 0, 1, 2
 
 _oracle = None
 _ref = None
+# "special_row" or "core": which path the last connected_components call took
+last_path = None
 
 
 class DimensionError(Exception):
@@ -56,6 +58,10 @@ def _lib_oracle():
       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
       ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
     ]
+    lib.oracle_statistics.restype = ctypes.c_int
+    lib.oracle_statistics.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.oracle_set_mutation.restype = None
     lib.oracle_set_mutation.argtypes = [ctypes.c_int]
     lib.oracle_epl.restype = ctypes.c_uint64
@@ -301,8 +307,13 @@ def connected_components(data, max_labels=-1, connectivity=26, return_N=False,
   flat = np.ravel(_view_unsigned(data), order="K")
   mode = MODE_BINARY if binary_image else (MODE_MULTI if delta == 0 else MODE_CONT)
 
+  global last_path
+  last_path = "special_row" if (delta == 0 and ffr == lfr and ffr >= 0) else "core"
   if delta == 0 and ffr == lfr and ffr >= 0:
-    out, N = _special_row(ffr, sx, sy, flat, out_dtype)
+    # The C++ core has no such path (the binding handles it in Python), so
+    # both backends use this restatement; tests/golden/special_row_vectors.npz
+    # (outputs of the reference's compiled binding) pins it.
+    out, N = _special_row(ffr, sx, sy, np.ravel(data, order="K"), out_dtype)
   elif backend == "ref":
     out, N = ref_core_label(flat, sx, sy, sz, max_labels, connectivity, delta,
                             out_dtype, periodic_boundary, binary_image)
@@ -327,9 +338,33 @@ def connected_components(data, max_labels=-1, connectivity=26, return_N=False,
   return (out, N) if return_N else out
 
 
+def _int64_of(val) -> int:
+  """Cython's conversion of a numpy scalar to `cdef int64_t`: int() of the
+  value (truncation for floats), then the C range check."""
+  if isinstance(val, np.floating):
+    f = float(val)
+    if f != f:
+      raise ValueError("cannot convert float NaN to integer")
+    if f in (float("inf"), float("-inf")):
+      raise OverflowError("cannot convert float infinity to integer")
+  i = int(val)
+  if i >= (1 << 63) or i < -(1 << 63):
+    raise OverflowError("Python int too large to convert to C long")
+  return i
+
+
 def _special_row(row: int, sx: int, sy: int, flat: np.ndarray, out_dtype):
-  """cc3d/fastcc3d.pyx:644-679 on integer rows: runs of equal nonzero values."""
+  """cc3d/fastcc3d.pyx:644-679: the single-foreground-row shortcut.
+
+  `flat` holds the binding's own `data` in memory order (signed and float
+  values as they are; float16 already viewed as uint16).  `last_label` is a C
+  int64: each new run stores the int64 conversion of its first value
+  (_int64_of, which raises like the reference), and a voxel continues the run
+  when it compares equal to that integer.  Each label is stored before the
+  conversion, so a label outside out_dtype raises numpy's OverflowError
+  first."""
   out = np.zeros(flat.size, dtype=out_dtype)
+  outmax = int(np.iinfo(out_dtype).max)
   start = row * sx
   N = 0
   last = 0
@@ -343,8 +378,10 @@ def _special_row(row: int, sx: int, sy: int, flat: np.ndarray, out_dtype):
       continue
     else:
       N += 1
+      if N > outmax:
+        raise OverflowError(f"Python integer {N} out of bounds for {np.dtype(out_dtype).name}")
       out[start + i] = N
-      last = val
+      last = _int64_of(val)
   return out, N
 
 
@@ -432,3 +469,39 @@ def voxel_connectivity_graph(data, connectivity=26, backend: str = "oracle"):
   elif dims == 2:
     return graph.reshape((sx, sy), order="F")
   return graph.reshape((sx,), order="F")
+
+
+# ---------------------------------------------------------------------------
+# statistics
+# ---------------------------------------------------------------------------
+
+def statistics_raw(labels: np.ndarray, N: Optional[int] = None):
+  """cc3d/fastcc3d.pyx:681-938 over oracle.c's oracle_statistics: (counts
+  uint32[N+1], bbox uint32[N+1, 2*ndim] as min,max per numpy axis (min
+  0xFFFFFFFF for absent labels), centroids float64[N+1, ndim] (NaN for absent
+  labels)).  labels: a 2D or 3D unsigned (or bool) array, C or F contiguous."""
+  lab = labels
+  if lab.dtype == np.bool_:
+    lab = lab.view(np.uint8)
+  if lab.dtype.kind == "i":
+    lab = lab.view(f"u{lab.dtype.itemsize}")
+  f_order = bool(lab.flags.f_contiguous)
+  if not (lab.flags.c_contiguous or f_order):
+    lab = np.ascontiguousarray(lab)
+  nd = lab.ndim
+  s = list(lab.shape) + [1] * (3 - nd)
+  if N is None:
+    N = int(lab.max())
+  counts = np.zeros(N + 1, dtype=np.uint32)
+  bbox = np.zeros((N + 1) * 6, dtype=np.uint32)
+  sums = np.zeros((N + 1) * 3, dtype=np.uint64)
+  rc = _lib_oracle().oracle_statistics(lab.ctypes.data, DT_CODE[lab.dtype], s[0], s[1], s[2], int(f_order), N,
+                                       counts.ctypes.data, bbox.ctypes.data, sums.ctypes.data)
+  if rc:
+    raise ValueError("label above N")
+  bbox = bbox.reshape(N + 1, 6)[:, :2 * nd]
+  sums = sums.reshape(N + 1, 3)[:, :nd].astype(np.float64)
+  with np.errstate(invalid="ignore", divide="ignore"):
+    cen = sums / counts[:, None].astype(np.float64)
+  cen[counts == 0] = np.nan
+  return counts, bbox, cen

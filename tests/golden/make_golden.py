@@ -3,14 +3,19 @@
 
 Outputs come from the reference implementation compiled from its own sources
 (oracle/_ref/libcc3d_ref.so, built by oracle/Makefile) called through the
-binding restatement in oracle/oracle.py.  Inputs are small seeded random
+binding restatement in oracle/oracle.py.  Cases that take the binding's
+single-foreground-row shortcut (cc3d/fastcc3d.pyx:469-470, :644-679), which
+is binding code and not part of the compiled core, are answered by the
+reference's binding itself, compiled outside the repository by
+make_special_row.build_binding.  Inputs are small seeded random
 volumes that exercise every path of the labelling core: multilabel, binary and
 continuous at every connectivity, periodic boundaries, both memory orders,
 all input dtypes, explicit output dtypes, and the quirk cases (binary 2D-8
 block numbering and stale labels on odd sides, binary 2D-4 row 0, continuous
 2D-8 range shortcut, union-find overflow).
 
-Run in a tree where oracle/_ref exists:  python tests/golden/make_golden.py
+Run in this container, where oracle/_ref and /root/reference exist:
+    python tests/golden/make_golden.py
 """
 import json
 import os
@@ -21,6 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
 import oracle  # noqa: E402
 
 DTYPES = ["uint8", "uint16", "int32", "uint32", "uint64", "int64", "float32", "float64", "int8", "int16",
@@ -62,20 +68,37 @@ def case(rng, i):
   return arr, order, kw
 
 
-def run_ref(arr, kw):
+def run_ref(arr, kw, binding=None):
+  """The reference's result: its C++ core compiled into oracle/_ref behind the
+  binding restatement, or -- for the single-foreground-row shortcut, which
+  lives in the binding rather than the core -- the reference's compiled
+  binding itself (make_special_row.build_binding)."""
   kw2 = dict(kw)
   if "out_dtype" in kw2:
     kw2["out_dtype"] = np.dtype(kw2["out_dtype"])
   try:
     out, N = oracle.connected_components(arr, return_N=True, backend="ref", **kw2)
-    return out, int(N), ""
+    res = (out, int(N), "")
   except Exception as e:
-    return None, -1, type(e).__name__
+    res = (None, -1, type(e).__name__)
+  if oracle.last_path == "special_row":
+    if binding is None:
+      raise SystemExit("single-row case needs the reference binding (tests/golden/make_special_row.py)")
+    try:
+      out, N = binding.connected_components(arr, return_N=True, **kw2)
+      res = (out, int(N), "")
+    except Exception as e:
+      res = (None, -1, type(e).__name__)
+  return res
 
 
 def main():
   if not oracle.have_ref():
     raise SystemExit("oracle/_ref/libcc3d_ref.so is missing: run `make -C oracle ref` where the reference exists")
+  import make_special_row
+  make_special_row.build_binding()
+  sys.path.insert(0, make_special_row.BUILD)
+  import fastcc3d as binding  # the reference's binding, built outside the repository
   rng = np.random.default_rng(20261202)
   blobs = {}
   meta = []
@@ -101,7 +124,7 @@ def main():
     extra.append((a, "C", {"connectivity": 8, "delta": int(rng.integers(20, 120))}))
   cases += extra
   for i, (arr, order, kw) in enumerate(cases):
-    out, N, err = run_ref(arr, kw)
+    out, N, err = run_ref(arr, kw, binding)
     blobs[f"in{i}"] = arr if arr.dtype != np.float16 else arr.view(np.uint16)
     if out is not None:
       blobs[f"out{i}"] = out

@@ -160,6 +160,83 @@ def test_dust_reference_known_answer(gpu):
     assert list(np.unique(rec)) == [0, 1] and N == 1
 
 
+def _dust_full_restated(img, threshold, connectivity=26, in_place=False, precomputed_ccl=False, invert=False):
+  """cc3d/__init__.py:101-155 step by step, with the oracle's labels and the
+  checks of the reference's statistics() (cc3d/fastcc3d.pyx:716-748)."""
+  orig = img.dtype
+  img = img.view(f"u{img.dtype.itemsize}") if img.dtype.kind == "i" else img
+  if not in_place:
+    img = np.copy(img)
+  if precomputed_ccl:
+    cc, N = img, int(np.max(img))
+  else:
+    cc, N = oracle.connected_components(img, connectivity=connectivity, return_N=True)
+  if N > cc.size:
+    raise ValueError(
+      f"Statistics can only be computed on volumes containing labels with values lower than the number of voxels. Max: {N}")
+  sizes = np.bincount(cc.ravel().astype(np.int64), minlength=N + 1)
+  if isinstance(threshold, (tuple, list)):
+    to_mask = [i for i in range(1, N + 1) if not (threshold[0] <= sizes[i] < threshold[1])]
+  else:
+    to_mask = [i for i in range(1, N + 1) if sizes[i] < threshold]
+  dust_N = len(to_mask) if invert else N - len(to_mask)
+  if not to_mask:
+    if invert:
+      img = np.zeros(img.shape, dtype=img.dtype, order="F")
+    return img, dust_N
+  mask = np.isin(cc, to_mask, invert=invert)
+  img[mask] = 0
+  return img.view(orig), dust_N
+
+
+@pytest.mark.parametrize("precomputed", [False, True])
+@pytest.mark.parametrize("invert", [False, True])
+def test_dust_in_place_strided_view(gpu, precomputed, invert):
+  """in_place=True on a non-contiguous view writes through to the caller's
+  array, as the reference's img[mask] = 0 does."""
+  import cc3d
+  rng = np.random.default_rng(17 + 2 * precomputed + invert)
+  base = rng.integers(0, 4, size=(50, 40, 30)).astype(np.uint16)
+  if precomputed:
+    lab, _ = oracle.connected_components(base, connectivity=26, return_N=True, out_dtype=np.uint32)
+    base = lab.astype(np.int32)
+  a = base.copy()
+  b = base.copy()
+  va, vb = a[:, ::2, 1:], b[:, ::2, 1:]
+  assert not (va.flags.c_contiguous or va.flags.f_contiguous)
+  got, gN = cc3d.dust(va, threshold=6, connectivity=26, in_place=True, precomputed_ccl=precomputed,
+                      invert=invert, return_N=True)
+  exp, eN = _dust_full_restated(vb, 6, 26, in_place=True, precomputed_ccl=precomputed, invert=invert)
+  assert gN == eN
+  assert got.dtype == exp.dtype and np.array_equal(got, exp)
+  assert np.array_equal(a, b), "the caller's array differs from the reference's in-place result"
+  # not in place: the caller's array is untouched
+  c = base.copy()
+  got2 = cc3d.dust(c[:, ::2, 1:], threshold=6, connectivity=26, precomputed_ccl=precomputed, invert=invert)
+  assert np.array_equal(c, base) and np.array_equal(got2, exp)
+
+
+def test_dust_precomputed_label_checks(gpu):
+  """precomputed_ccl runs the reference statistics() check: a label above the
+  voxel count (a negative signed label viewed as unsigned included) raises
+  ValueError with the reference's message."""
+  import cc3d
+  img = np.zeros((4, 4, 4), dtype=np.uint32)
+  img[0, 0, 0] = 1000
+  with pytest.raises(ValueError) as e1:
+    cc3d.dust(img, threshold=3, precomputed_ccl=True)
+  with pytest.raises(ValueError) as e2:
+    _dust_full_restated(img, 3, precomputed_ccl=True)
+  assert str(e1.value) == str(e2.value)
+  neg = np.zeros((4, 4, 4), dtype=np.int16)
+  neg[1, 2, 3] = -2
+  with pytest.raises(ValueError) as e1:
+    cc3d.dust(neg, threshold=3, precomputed_ccl=True)
+  with pytest.raises(ValueError) as e2:
+    _dust_full_restated(neg, 3, precomputed_ccl=True)
+  assert str(e1.value) == str(e2.value)
+
+
 def _largest_k_restated(img, k, connectivity=26, delta=0, return_N=False, binary_image=False,
                         precomputed_ccl=False):
   """cc3d/__init__.py:199-279 without fastremap (absent here, as in a plain

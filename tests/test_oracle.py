@@ -303,3 +303,79 @@ def test_vcg_oracle_matches_reference_build():
       exp = oracle.voxel_connectivity_graph(arr, connectivity=conn, backend="ref")
       got = oracle.voxel_connectivity_graph(arr, connectivity=conn)
       assert got.dtype == exp.dtype and got.shape == exp.shape and np.array_equal(got, exp), (shape, conn, dt)
+
+
+def test_oracle_special_row_matches_reference_binding():
+  """The single-foreground-row shortcut (cc3d/fastcc3d.pyx:644-679) against
+  the reference's compiled binding: float truncation of the run value, NaN /
+  infinity / >= 2^63 conversion errors, uint16 overflow."""
+  import special_row_cases as sr
+  cases = sr.load()
+  assert len(cases) >= 70 and sum(1 for c in cases if c[4]) >= 20
+  for i, (arr, kw, out, N, et, em) in enumerate(cases):
+    sr.check(lambda a, **k: oracle.connected_components(a, backend="oracle", **k), arr, kw, out, N, et, em,
+             f"special row {i}")
+
+
+def test_special_row_vectors_catch_planted_mistake(monkeypatch):
+  """Labelling the row as runs of equal values (no int64 conversion) is
+  caught by the vectors."""
+  import special_row_cases as sr
+  monkeypatch.setattr(oracle, "_int64_of", lambda v: v)
+  bad = 0
+  for arr, kw, out, N, et, em in sr.load():
+    try:
+      sr.check(lambda a, **k: oracle.connected_components(a, backend="oracle", **k), arr, kw, out, N, et, em, "")
+    except AssertionError:
+      bad += 1
+  assert bad >= 20
+
+
+def _np_statistics(lab):
+  """numpy closed forms: bincount counts, per-label min/max of coordinates,
+  coordinate sums / counts."""
+  N = int(lab.max())
+  nd = lab.ndim
+  counts = np.bincount(lab.ravel().astype(np.int64), minlength=N + 1).astype(np.uint32)
+  idx = np.nonzero(np.ones(lab.shape, bool))
+  flat = lab.ravel()
+  bb = np.zeros((N + 1, 2 * nd), dtype=np.int64)
+  cen = np.full((N + 1, nd), np.nan)
+  for a in range(nd):
+    mn = np.full(N + 1, 0xFFFFFFFF, np.int64)
+    mx = np.zeros(N + 1, np.int64)
+    np.minimum.at(mn, flat, idx[a])
+    np.maximum.at(mx, flat, idx[a])
+    bb[:, 2 * a], bb[:, 2 * a + 1] = mn, mx
+    s = np.zeros(N + 1, np.int64)
+    np.add.at(s, flat, idx[a])
+    with np.errstate(invalid="ignore", divide="ignore"):
+      cen[:, a] = np.where(counts == 0, np.nan, s / counts)
+  return counts, bb, cen
+
+
+@pytest.mark.parametrize("order", ["C", "F"])
+@pytest.mark.parametrize("shape,dtype", [((40, 33, 21), np.uint32), ((70, 45), np.uint16), ((17, 1, 9), np.uint8),
+                                         ((9, 300), np.uint64)])
+def test_oracle_statistics_matches_numpy(order, shape, dtype):
+  rng = np.random.default_rng(sum(shape))
+  lab = rng.integers(0, 30, size=shape).astype(dtype)
+  lab[lab == 7] = 0  # an absent label
+  lab = np.asfortranarray(lab) if order == "F" else np.ascontiguousarray(lab)
+  c, b, cen = oracle.statistics_raw(lab)
+  c2, b2, cen2 = _np_statistics(lab)
+  assert np.array_equal(c, c2) and np.array_equal(b.astype(np.int64), b2)
+  assert np.array_equal(np.isnan(cen), np.isnan(cen2)) and np.array_equal(np.nan_to_num(cen), np.nan_to_num(cen2))
+
+
+def test_oracle_statistics_reference_known_answer():
+  """cc3d's own statistics test values (automated_test.py:962-1006)."""
+  for order in ("C", "F"):
+    labels = np.zeros((123, 128, 125), dtype=np.uint8, order=order)
+    labels[10:20, 10:20, 10:20] = 1
+    labels[40:50, 40:50, 40:51] = 2
+    c, b, cen = oracle.statistics_raw(labels)
+    assert c[1] == 1000 and c[2] == 10 * 10 * 11
+    assert np.allclose(cen[1], [14.5, 14.5, 14.5]) and np.allclose(cen[2], [44.5, 44.5, 45])
+    assert list(b[0]) == [0, 122, 0, 127, 0, 124]
+    assert list(b[1]) == [10, 19, 10, 19, 10, 19] and list(b[2]) == [40, 49, 40, 49, 40, 50]
