@@ -283,7 +283,8 @@ def configs_leg(torch, cc3d, synth, dev, steps=5):
   """The other workloads at full size: GPU throughput with the input resident
   in HBM, the reference on one host core, and a bit-exact check of the two."""
   import oracle
-  cases = []
+  cases = [("256^3 uint32 Voronoi 1500 seeds 26-conn", lambda: synth.voronoi((256, 256, 256), 1500, seed=2024),
+            dict(connectivity=26, out_dtype=np.uint32))]
   for p in (0.1, 0.3, 0.5):
     for conn in (6, 26):
       cases.append((f"512^3 uint8 Bernoulli p={p} binary {conn}-conn",
@@ -316,8 +317,10 @@ def configs_leg(torch, cc3d, synth, dev, steps=5):
       out, N = cc3d.connected_components(d, return_N=True, **kw)
     torch.cuda.synchronize()
     gpu_s = (time.perf_counter() - t0) / steps
+    path_gbs = (vol.itemsize + 4) * V / gpu_s / 1e9
     row = {"workload": name, "voxels": V, "gvox_s": V / gpu_s / 1e9, "ms": gpu_s * 1e3, "components": int(N),
-           "path_gbs": (vol.itemsize + 4) * V / gpu_s / 1e9}
+           "path_gbs": path_gbs, "roofline_frac": path_gbs / HBM_PEAK_GBS,
+           "algorithmic_bytes_per_voxel": vol.itemsize + 4}
     if oracle.have_ref():
       t0 = time.perf_counter()
       ref, rN = oracle.connected_components(vol, return_N=True, backend="ref", **kw)
@@ -338,8 +341,8 @@ def main() -> int:
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
   ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
   ap.add_argument("--no-served", action="store_true", help="skip the served leg")
-  ap.add_argument("--configs", action="store_true",
-                  help="also time the other workloads at full size against the reference")
+  ap.add_argument("--no-configs", action="store_true",
+                  help="skip timing the other workloads at full size against the reference")
   ap.add_argument("--calib", action="store_true",
                   help="also run one 256 MiB device copy (counter calibration for profiles)")
   args = ap.parse_args()
@@ -459,7 +462,8 @@ def main() -> int:
     batched = batched_leg(torch, cc3d, synth, dev, cpu=not args.no_cpu)
 
   configs = None
-  if args.configs:
+  if not args.no_configs:
+    torch.cuda.empty_cache()
     configs = configs_leg(torch, cc3d, synth, dev)
 
   served = None

@@ -8,7 +8,7 @@ signatures and results, computed by hand-written HIP kernels for gfx950
 from ._batch import LabelBatch, connected_components_batch
 from ._core import DimensionError, connected_components, estimate_provisional_labels
 from ._follow import dust, largest_k, statistics
-from ._graphs import voxel_connectivity_graph
+from ._graphs import color_connectivity_graph, voxel_connectivity_graph
 from ._serve import LabelServer, Request
 
 __version__ = "4.0.0+mi355x.0.1"
@@ -20,6 +20,7 @@ __all__ = [
   "dust",
   "largest_k",
   "voxel_connectivity_graph",
+  "color_connectivity_graph",
   "DimensionError",
   "connected_components_batch",
   "LabelBatch",

@@ -219,6 +219,19 @@ int cc3d_amd_voxel_connectivity_graph(const void* labels, int dtype, int64_t sx,
                                       int connectivity, void* out, void* stream);
 const char* cc3d_amd_graphs_last_error(void);
 
+/* Replaces color_connectivity_graph_N<VCG_t, uint32_t>
+ * (cc3d/cc3d_graphs.hpp:1077-1106, bound at cc3d/fastcc3d.pyx:941-1018):
+ * connected components of a voxel connectivity graph.  vcg: device, x
+ * fastest, uint8 (vcg_size 1) or uint32 (4) with the bit layout of
+ * cc3d_amd_voxel_connectivity_graph; every voxel is labelled (1..N, numbered
+ * by first voxel in raster order); voxel p joins its backward neighbour q
+ * when p's bit for the direction p -> q is set.  connectivity 26 or 6 for
+ * sz > 1 (26 needs a uint32 graph), 8 or 4 (6 and 26 read as 4 and 8) for
+ * sz == 1.  out: device uint32 (sx*sy*sz).  Waits for the stream. */
+int cc3d_amd_color_connectivity_graph(const void* vcg, int vcg_size, int64_t sx, int64_t sy, int64_t sz,
+                                      int connectivity, uint32_t* out, uint64_t* N, void* stream);
+const char* cc3d_amd_color_last_error(void);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  No
  * reference counterpart (measurement support).  on: 0 off, 1 every kernel,
  * 2 only the local labelling kernel.  collect() waits for the events and

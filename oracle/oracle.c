@@ -597,3 +597,106 @@ int oracle_statistics(const void* labels, int dt, int64_t s0, int64_t s1, int64_
   }
   return 0;
 }
+
+/*
+ * oracle_color_vcg: cc3d.color_connectivity_graph restated
+ * (cc3d/cc3d_graphs.hpp:525-1106, bound at cc3d/fastcc3d.pyx:941-1018).
+ * Every voxel is a node; voxel p is joined to a backward neighbour q when
+ * p's own bit for the direction p -> q is set and q lies in the volume (the
+ * graph is taken as undirected, so q's bits are not read).  Backward
+ * neighbours per variant (the reference's first and second passes):
+ *   3D 26 (uint32): (-1,0,0) (-1,-1,0) (0,-1,0) (+1,-1,0) and the 9 voxels
+ *                   of plane z - 1;
+ *   3D 6:           (-1,0,0) (0,-1,0) (0,0,-1);
+ *   2D 8:           (-1,0) (-1,-1) (0,-1) (+1,-1);
+ *   2D 4:           (-1,0) (0,-1).
+ * Bit of a direction: the voxel_connectivity_graph layout -- for uint32 the
+ * 3D one (in 2D the in-plane diagonals are bits 6-9), for uint8 in 2D bits
+ * 4-7 for the diagonals.  Components are numbered by their first voxel in
+ * raster order (x fastest), the order in which the reference creates and
+ * then renumbers its provisional labels.  variant: 0 = 3D 26, 1 = 3D 6,
+ * 2 = 2D 8, 3 = 2D 4.  Returns N.
+ */
+static int vcg_bit3(int dx, int dy, int dz) {
+  static const int8_t dirs[26][3] = {
+    {+1, 0, 0},   {-1, 0, 0},   {0, +1, 0},   {0, -1, 0},   {0, 0, +1},   {0, 0, -1},   {+1, +1, 0},
+    {-1, +1, 0},  {+1, -1, 0},  {-1, -1, 0},  {+1, 0, +1},  {-1, 0, +1},  {0, +1, +1},  {0, -1, +1},
+    {+1, 0, -1},  {-1, 0, -1},  {0, +1, -1},  {0, -1, -1},  {+1, +1, +1}, {-1, +1, +1}, {+1, -1, +1},
+    {-1, -1, +1}, {+1, +1, -1}, {-1, +1, -1}, {+1, -1, -1}, {-1, -1, -1},
+  };
+  for (int i = 0; i < 26; i++)
+    if (dirs[i][0] == dx && dirs[i][1] == dy && dirs[i][2] == dz) return i;
+  return -1;
+}
+static int vcg_bit2(int dx, int dy) {
+  static const int8_t dirs[8][2] = {{+1, 0}, {-1, 0}, {0, +1}, {0, -1}, {+1, +1}, {-1, +1}, {+1, -1}, {-1, -1}};
+  for (int i = 0; i < 8; i++)
+    if (dirs[i][0] == dx && dirs[i][1] == dy) return i;
+  return -1;
+}
+
+uint64_t oracle_color_vcg(const void* vcg, int vcg_size, int64_t sx, int64_t sy, int64_t sz, int variant,
+                          uint32_t* out) {
+  const int64_t V = sx * sy * sz;
+  int nb = 0;
+  int off[13][3], bit[13];
+  if (variant == 0) {
+    const int in_plane[4][2] = {{-1, 0}, {-1, -1}, {0, -1}, {+1, -1}};
+    for (int i = 0; i < 4; i++) {
+      off[nb][0] = in_plane[i][0]; off[nb][1] = in_plane[i][1]; off[nb][2] = 0;
+      bit[nb] = vcg_bit3(in_plane[i][0], in_plane[i][1], 0);
+      nb++;
+    }
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        off[nb][0] = dx; off[nb][1] = dy; off[nb][2] = -1;
+        bit[nb] = vcg_bit3(dx, dy, -1);
+        nb++;
+      }
+  } else if (variant == 1) {
+    const int o[3][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    for (int i = 0; i < 3; i++) {
+      off[nb][0] = o[i][0]; off[nb][1] = o[i][1]; off[nb][2] = o[i][2];
+      bit[nb] = vcg_bit3(o[i][0], o[i][1], o[i][2]);
+      nb++;
+    }
+  } else {
+    const int o8[4][2] = {{-1, 0}, {-1, -1}, {0, -1}, {+1, -1}};
+    const int n = variant == 2 ? 4 : 2;
+    const int o4[2][2] = {{-1, 0}, {0, -1}};
+    for (int i = 0; i < n; i++) {
+      const int dx = variant == 2 ? o8[i][0] : o4[i][0], dy = variant == 2 ? o8[i][1] : o4[i][1];
+      off[nb][0] = dx; off[nb][1] = dy; off[nb][2] = 0;
+      bit[nb] = vcg_size == 1 ? vcg_bit2(dx, dy) : vcg_bit3(dx, dy, 0);
+      nb++;
+    }
+  }
+  int64_t* par = (int64_t*)malloc((size_t)V * sizeof(int64_t));
+  for (int64_t i = 0; i < V; i++) par[i] = i;
+  for (int64_t z = 0; z < sz; z++)
+    for (int64_t y = 0; y < sy; y++)
+      for (int64_t x = 0; x < sx; x++) {
+        const int64_t p = x + sx * (y + sy * z);
+        const uint32_t bits = vcg_size == 1 ? ((const uint8_t*)vcg)[p] : ((const uint32_t*)vcg)[p];
+        for (int k = 0; k < nb; k++) {
+          const int64_t qx = x + off[k][0], qy = y + off[k][1], qz = z + off[k][2];
+          if (qx < 0 || qx >= sx || qy < 0 || qy >= sy || qz < 0 || qz >= sz) continue;
+          if (!((bits >> bit[k]) & 1u)) continue;
+          int64_t a = p, b = qx + sx * (qy + sy * qz);
+          while (par[a] != a) a = par[a] = par[par[a]];
+          while (par[b] != b) b = par[b] = par[par[b]];
+          if (a == b) continue;
+          if (a < b) par[b] = a;
+          else par[a] = b;
+        }
+      }
+  uint64_t N = 0;
+  for (int64_t i = 0; i < V; i++) {
+    int64_t r = i;
+    while (par[r] != r) r = par[r];
+    if (r == i) out[i] = (uint32_t)(++N);
+    else out[i] = out[r];
+  }
+  free(par);
+  return N;
+}

@@ -505,3 +505,65 @@ def statistics_raw(labels: np.ndarray, N: Optional[int] = None):
     cen = sums / counts[:, None].astype(np.float64)
   cen[counts == 0] = np.nan
   return counts, bbox, cen
+
+
+# ---------------------------------------------------------------------------
+# color_connectivity_graph
+# ---------------------------------------------------------------------------
+
+def color_connectivity_graph(vcg, connectivity=26, return_N=False, backend: str = "oracle"):
+  """Checker-side cc3d.color_connectivity_graph (cc3d/fastcc3d.pyx:941-1018):
+  the binding restated in Python over oracle.c's oracle_color_vcg, or over
+  the reference's own color_connectivity_graph_N (oracle/_ref)."""
+  vcg = np.asarray(vcg)
+  dims = vcg.ndim
+  if dims not in (2, 3):
+    raise DimensionError("Only 2D, and 3D arrays supported. Got: " + str(dims))
+  if dims == 2 and connectivity not in [4, 8, 6, 26]:
+    raise ValueError(f"Only 4 and 8 connectivity is supported for 2D images. Got: {connectivity}")
+  elif dims != 2 and connectivity not in [6, 26]:
+    raise ValueError(f"Only 6 and 26 connectivity are supported for 3D images. Got: {connectivity}")
+  dtype = vcg.dtype
+  if dtype not in [np.uint8, np.uint32]:
+    raise ValueError(f"Only uint8 and uint32 are supported. Got: {vcg.dtype}")
+  if vcg.size == 0:
+    return np.zeros([0] * dims, dtype=np.uint32, order="F")
+  while vcg.ndim < 3:
+    vcg = vcg[..., np.newaxis]
+  vcg = np.asfortranarray(vcg)
+  sx, sy, sz = vcg.shape
+  if connectivity in [18, 26] and sz > 1 and dtype != np.uint32:
+    raise ValueError(f"Only uint32 is supported for 18 and 26 connected. Got: {vcg.dtype}")
+  flat = np.ravel(vcg, order="F")
+  out = np.zeros(flat.size, dtype=np.uint32)
+  if backend == "ref":
+    lib = _lib_ref()
+    lib.ref_color_connectivity_graph.restype = ctypes.c_int
+    lib.ref_color_connectivity_graph.argtypes = [
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+      ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_int]
+    n = ctypes.c_uint64(0)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.ref_color_connectivity_graph(flat.ctypes.data, flat.itemsize, sx, sy, sz, int(connectivity),
+                                          out.ctypes.data, ctypes.byref(n), err, 256)
+    if rc != 0:
+      raise RuntimeError(err.value.decode())
+    N = int(n.value)
+  else:
+    if connectivity not in (26, 6, 4, 8):
+      raise RuntimeError("Only 4, 8, 6 and 26 connectivities are supported.")
+    if sz > 1 and connectivity not in (6, 26):
+      raise RuntimeError("Only 6 and 26 connectivity is supported in 3D.")
+    if sz == 1:
+      variant = 3 if connectivity in (6, 4) else 2
+    else:
+      variant = 1 if connectivity == 6 else 0
+    lib = _lib_oracle()
+    lib.oracle_color_vcg.restype = ctypes.c_uint64
+    lib.oracle_color_vcg.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int, ctypes.c_void_p]
+    N = int(lib.oracle_color_vcg(flat.ctypes.data, flat.itemsize, sx, sy, sz, variant, out.ctypes.data))
+  out = out.reshape((sx, sy, sz), order="F")
+  while out.ndim > dims:
+    out = out[..., 0]
+  return (out, N) if return_N else out

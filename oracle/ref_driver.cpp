@@ -140,4 +140,35 @@ int ref_voxel_connectivity_graph(const void* in, int in_dtype, int64_t sx, int64
   return 0;
 }
 
+// cc3d/cc3d_graphs.hpp:1077-1106 (color_connectivity_graph_N) through
+// cc3d/fastcc3d.pyx:941-1018: vcg is uint8 (vcg_size 1) or uint32 (4), x
+// fastest; out is zeroed uint32.  Returns 0, or 1 on a C++ exception
+// (message in errbuf).
+int ref_color_connectivity_graph(const void* vcg, int vcg_size, int64_t sx, int64_t sy, int64_t sz,
+                                 int connectivity, uint32_t* out, uint64_t* N, char* errbuf, int errlen) {
+  try {
+    size_t n = 0;
+    if (vcg_size == 1)
+      cc3d::color_connectivity_graph_N<uint8_t, uint32_t>(static_cast<const uint8_t*>(vcg), sx, sy, sz, connectivity,
+                                                          out, n);
+    else
+      cc3d::color_connectivity_graph_N<uint32_t, uint32_t>(static_cast<const uint32_t*>(vcg), sx, sy, sz,
+                                                           connectivity, out, n);
+    *N = n;
+  } catch (const std::exception& e) {
+    if (errbuf && errlen > 0) {
+      std::strncpy(errbuf, e.what(), errlen - 1);
+      errbuf[errlen - 1] = 0;
+    }
+    return 1;
+  } catch (...) {  // the reference throws a pointer for 26 with a uint8 graph
+    if (errbuf && errlen > 0) {
+      std::strncpy(errbuf, "26-connectivity requires a 32-bit voxel graph for color_connectivity_graph_26.", errlen - 1);
+      errbuf[errlen - 1] = 0;
+    }
+    return 1;
+  }
+  return 0;
+}
+
 }  // extern "C"

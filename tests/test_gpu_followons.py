@@ -360,3 +360,53 @@ def test_vcg_errors(gpu):
   with pytest.raises(cc3d.DimensionError):
     cc3d.voxel_connectivity_graph(np.zeros((2, 2, 2, 2), np.uint8))
   assert cc3d.voxel_connectivity_graph(np.zeros((0, 3), np.uint8), connectivity=4).shape == (0,)
+
+
+def test_color_graph_reference_known_answers(gpu):
+  """automated_test.py:1436-1568 through the product."""
+  import cc3d
+  import ccg_known_answers as ka
+  ka.check_all(cc3d.color_connectivity_graph)
+
+
+def test_color_graph_matches_oracle(gpu):
+  import cc3d
+  import ccg_known_answers as ka
+  rng = np.random.default_rng(11)
+  for i, (vcg, conn) in enumerate(ka.random_cases(rng, 120)):
+    print("colour case", i, vcg.shape, vcg.dtype, conn, flush=True)
+    got, gn = cc3d.color_connectivity_graph(vcg, connectivity=conn, return_N=True)
+    exp, en = oracle.color_connectivity_graph(vcg, connectivity=conn, return_N=True)
+    assert got.dtype == exp.dtype and got.shape == exp.shape and got.flags.f_contiguous
+    assert gn == en and np.array_equal(got, exp), i
+    if oracle.have_ref() and i % 4 == 0:
+      assert np.array_equal(got, oracle.color_connectivity_graph(vcg, connectivity=conn, backend="ref"))
+
+
+@pytest.mark.parametrize("conn", [6, 26])
+def test_color_graph_full_size_round_trip(gpu, conn):
+  """A 256^3 label volume with no background: the colouring of its own
+  connectivity graph is its connected-components labelling (both number by
+  first voxel in F order), and it equals the oracle's."""
+  import cc3d
+  import synth
+  lab = np.asfortranarray(synth.voronoi((256, 256, 256), 3000, seed=9).astype(np.uint32) + 1)
+  lab[::7, ::5, ::3] = 5  # broken-up regions
+  vcg = cc3d.voxel_connectivity_graph(lab, connectivity=conn)
+  got, gn = cc3d.color_connectivity_graph(vcg, connectivity=conn, return_N=True)
+  cc, cn = cc3d.connected_components(lab, connectivity=conn, return_N=True)
+  assert gn == cn and np.array_equal(got, cc)
+  exp, en = oracle.color_connectivity_graph(vcg, connectivity=conn, return_N=True)
+  assert en == gn and np.array_equal(got, exp)
+
+
+def test_color_graph_errors(gpu):
+  import cc3d
+  with pytest.raises(cc3d.DimensionError):
+    cc3d.color_connectivity_graph(np.zeros((2, 2, 2, 2), np.uint32))
+  with pytest.raises(ValueError):
+    cc3d.color_connectivity_graph(np.zeros((4, 4), np.uint8), connectivity=18)
+  with pytest.raises(ValueError):
+    cc3d.color_connectivity_graph(np.zeros((4, 4, 4), np.uint32), connectivity=8)
+  with pytest.raises(ValueError):
+    cc3d.color_connectivity_graph(np.zeros((4, 4), np.uint16), connectivity=4)

@@ -379,3 +379,34 @@ def test_oracle_statistics_reference_known_answer():
     assert np.allclose(cen[1], [14.5, 14.5, 14.5]) and np.allclose(cen[2], [44.5, 44.5, 45])
     assert list(b[0]) == [0, 122, 0, 127, 0, 124]
     assert list(b[1]) == [10, 19, 10, 19, 10, 19] and list(b[2]) == [40, 49, 40, 49, 40, 50]
+
+
+def test_oracle_color_graph_known_answers():
+  """color_connectivity_graph: the reference's own cases (automated_test.py:1436-1568)."""
+  import ccg_known_answers as ka
+  ka.check_all(lambda v, **k: oracle.color_connectivity_graph(v, backend="oracle", **k))
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built")
+def test_oracle_color_graph_matches_compiled_reference():
+  import ccg_known_answers as ka
+  rng = np.random.default_rng(5)
+  for vcg, conn in ka.random_cases(rng, 250):
+    a, na = oracle.color_connectivity_graph(vcg, connectivity=conn, return_N=True, backend="ref")
+    b, nb = oracle.color_connectivity_graph(vcg, connectivity=conn, return_N=True, backend="oracle")
+    assert na == nb and a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b), (vcg.shape, conn)
+  ka.check_all(lambda v, **k: oracle.color_connectivity_graph(v, backend="ref", **k))
+
+
+def test_color_graph_checks_catch_planted_mistake(monkeypatch):
+  """Reading q's bits instead of p's is caught: on a directed graph (bits set
+  on one side only) the partition changes."""
+  import ccg_known_answers as ka
+  vcg = np.zeros((4, 3), dtype=np.uint8)
+  vcg[1, 0] = 0b10  # voxel (1, 0) joins (0, 0); (0, 0) has no +x bit
+  a, n = oracle.color_connectivity_graph(vcg, connectivity=4, return_N=True)
+  assert n == 11 and a[0, 0] == a[1, 0]
+  flipped = np.zeros((4, 3), dtype=np.uint8)
+  flipped[0, 0] = 0b01  # only the +x side set: no join
+  b, m = oracle.color_connectivity_graph(flipped, connectivity=4, return_N=True)
+  assert m == 12
