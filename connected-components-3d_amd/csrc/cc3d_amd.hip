@@ -1005,6 +1005,32 @@ static int utab_ready(const uint16_t** out, hipStream_t on = nullptr) {
   return CC3D_AMD_OK;
 }
 
+// Scratch of the no-allocation entry point: one copy per device, held until
+// the process ends.  Host code never passes the array's name to a kernel
+// (in host code it is only a stand-in for the device address): every use goes
+// through the address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) unsigned char scratch[CC3D_AMD_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::map<int, char*> g_scratch_dev;
+
+static int scratch_ready(char** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_dev.find(dev);
+  if (it != g_scratch_dev.end()) {
+    *out = it->second;
+    return CC3D_AMD_OK;
+  }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(CC3D_AMD_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+  g_scratch_dev[dev] = static_cast<char*>(ptr);
+  *out = static_cast<char*>(ptr);
+  return CC3D_AMD_OK;
+}
+
 // Optional per-kernel timing with HIP events on the launch stream (enabled by
 // cc3d_amd_profile_enable; read back by cc3d_amd_profile_collect).
 struct ProfRec {
@@ -1703,6 +1729,31 @@ int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, in
                                     void* stream) {
   return cc_core(in, in_dtype, sx, sy, sz, max_labels, connectivity, delta, out, out_dtype, N, periodic_boundary,
                  binary_image, stream, nullptr, 0);
+}
+
+int cc3d_amd_connected_components3d_noalloc(const void* in, int in_dtype, int64_t sx, int64_t sy, int64_t sz,
+                                            int64_t max_labels, int64_t connectivity, const void* delta, void* out,
+                                            int out_dtype, uint64_t* N, int periodic_boundary, int binary_image,
+                                            void* stream) {
+  Geometry g;
+  uint64_t db = 0;
+  delta_bits_of(in_dtype, delta, db);
+  const bool continuous = !binary_image && !delta_is_zero(in_dtype, db);
+  if (sx > 0 && sy > 0 && sz > 0 && plan(g, sx, sy, sz, connectivity, binary_image, continuous) == CC3D_AMD_OK &&
+      g.total > CC3D_AMD_NOALLOC_SCRATCH_BYTES)
+    return fail(CC3D_AMD_ERR_INVALID, "cc3d_amd_connected_components3d_noalloc: the volume needs " +
+                                          std::to_string(g.total) + " bytes of scratch; the limit is " +
+                                          std::to_string((unsigned long long)CC3D_AMD_NOALLOC_SCRATCH_BYTES));
+  char* ptr = nullptr;
+  int rc = scratch_ready(&ptr);
+  if (rc != CC3D_AMD_OK) return rc;
+  // per-kernel timing would create events: off for this call
+  const bool was_suspended = g_prof_suspended;
+  g_prof_suspended = true;
+  rc = cc_core(in, in_dtype, sx, sy, sz, max_labels, connectivity, delta, out, out_dtype, N, periodic_boundary,
+               binary_image, stream, ptr, CC3D_AMD_NOALLOC_SCRATCH_BYTES);
+  g_prof_suspended = was_suspended;
+  return rc;
 }
 
 void cc3d_amd_profile_enable(int on) {

@@ -45,6 +45,22 @@ int cc3d_amd_connected_components3d(const void* in, int in_dtype, int64_t sx, in
                                     const void* delta, void* out, int out_dtype, uint64_t* N,
                                     int periodic_boundary, int binary_image, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cc3d_amd_connected_components3d with the same arguments
+ * and results, but every workspace byte lives in a fixed device array of
+ * CC3D_AMD_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), and
+ * the call makes no hipMalloc/hipFree or other allocating call.  Volumes
+ * whose workspace (cc3d_amd_workspace_bytes) exceeds it are rejected with
+ * status 2, e.g. 256^3 voxels fit.  All calls on one GPU share that array,
+ * so calls that overlap in time on one GPU race: serialise them.  The call
+ * returns after its work on `stream` has finished. */
+#define CC3D_AMD_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int cc3d_amd_connected_components3d_noalloc(const void* in, int in_dtype, int64_t sx, int64_t sy,
+                                            int64_t sz, int64_t max_labels, int64_t connectivity,
+                                            const void* delta, void* out, int out_dtype, uint64_t* N,
+                                            int periodic_boundary, int binary_image, void* stream);
+
 /* Replaces cc3d::estimate_provisional_label_count<T>(T* in, sx, voxels,
  * int64& first_foreground_row, int64& last_foreground_row) --
  * cc3d/cc3d.hpp:287-315, bound at cc3d/fastcc3d.pyx:60-65. */
