@@ -412,7 +412,12 @@ struct LocalLds {
   alignas(4) uint16_t par[(G::NIDS_MAX + 1) & ~1];  // pmin works on aligned 32-bit words
   // storage of steps that are never live together, separated by barriers
   union {
-    uint16_t hqueue[G::IS3D ? G::HALO : 1];  // steps 1-2: box indices of halo run starts
+    struct {  // steps 1-3
+      uint16_t hqueue[G::IS3D ? G::HALO : 1];  // box indices of halo run starts
+      // per ring column (3D): bit k set unless its voxel k is linked to voxel
+      // k - 1 (bit 0 always set)
+      uint8_t rbreak[G::IS3D ? G::RING : 1];
+    } h;
     struct {
       uint16_t queue[QC];  // steps 3-4: voxel position | run member << 15
       uint32_t rbits[RW];  // steps 5-8: local roots that own tile voxels
@@ -421,10 +426,11 @@ struct LocalLds {
       uint32_t rprefix[RW];  // exclusive popcount prefix of rbits
       uint32_t red_epl[NW], red_prov[NW];
       int32_t red_first[NW], red_last[NW];
+      uint32_t nroots;
     } late;
   };
-  uint32_t qcount, nreq, hcount, nroots;
-  static_assert(QC % 2 == 0 && RW * 4 + NW * 16 <= QC * 2, "the late storage stays clear of rbits");
+  uint32_t qcount, nreq, hcount;
+  static_assert(QC % 2 == 0 && RW * 4 + NW * 16 + 4 <= QC * 2, "the late storage stays clear of rbits");
 };
 
 // Labels tile `tile` of the volume described by p.  Per-tile outputs go to
@@ -453,6 +459,9 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   static_assert(RW <= 128, "one wave scans the root bitmask");
   // 3D transitive 26/18: halo staged as runs (see step 1)
   constexpr bool HRUNS = PIECES && IS3D && MODE != M_CONT && (NB == 26 || NB == 18);
+  // 26-connected border run members consult the ring columns' run breaks (step 3)
+  constexpr bool RBREAK = HRUNS && NB == 26;
+  static_assert(!RBREAK || G::TZ <= 8, "one byte of run breaks per ring column");
   constexpr uint32_t NONE = 0xFFFFFFFFu;
 
   T* sv = L.sv;
@@ -462,7 +471,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   const uint16_t* ut0 = UT ? p.utab + UTab<UT ? NB : 26>::OFF : nullptr;
   uint16_t* queue = L.w.queue;
   constexpr uint32_t QC = LocalLds<T, G, FULLQ>::QC;
-  uint16_t* hqueue = L.hqueue;
+  uint16_t* hqueue = L.h.hqueue;
   uint32_t& qcount = L.qcount;
   uint32_t& nreq = L.nreq;
   uint32_t& hcount = L.hcount;
@@ -687,6 +696,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       // before it (staged by the previous segment's thread)
       T prev = (NBS > 1 && !ring) ? hpv : T(0);
       uint32_t start = (NBS > 1 && !ring) ? (uint32_t)G::box(cbx, hby0 > 0 ? hby0 - 1 : 0, 0) : 0u;
+      uint32_t brk = 0;  // ring column: bit k = voxel k is not linked to voxel k - 1
 #pragma unroll
       for (int k = 0; k < HLEN; k++) {
         if (k >= hlen) break;
@@ -697,13 +707,15 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         // predecessor in the same neighbouring tile: along z for the ring,
         // rows 1..TY of the bottom plane (the tile below) along y
         const bool linkable = ring ? k >= 1 : (by >= 2 && by <= G::TY);
-        const bool member = fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax) &&
-                            (ring || by < G::TY ? cov_a : cov_b);
+        const bool linked = fg(v) && linkable && column_link<T, MODE, NB, QK>(v, prev, delta, gmin, gmax);
+        const bool member = linked && (ring || by < G::TY ? cov_a : cov_b);
         if (!member) start = (uint32_t)b;
         par[b] = start;
         if (fg(v) && !member) hneed |= 1u << k;
+        if (!linked) brk |= 1u << k;
         prev = v;
       }
+      if (RBREAK && ring) L.h.rbreak[t] = (uint8_t)brk;
     }
     // queue the run starts: a wave-level scan and one LDS atomic per wave
     const uint32_t hc = (uint32_t)__popc(hneed);
@@ -784,6 +796,33 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     constexpr uint32_t ALLN = (1u << Nbhd<NB>::N) - 1u;
     const bool covers_all0 = (cover_from<NB>(0, bd0) | 1u) == ALLN;
     const bool covers_all = (cover_from<NB>(0, bdc) | 1u) == ALLN;
+    // 26-connected run member p at k >= 1 of a border column: the positions
+    // its column link leaves uncovered are halo voxels q of ring columns.  If
+    // q is linked to the voxel q' below it in its ring column (so q ~ q'),
+    // then q' is linked to p's predecessor E at the same offset (values are
+    // equal), E ~ q' holds by the same argument one voxel down (or by E's
+    // own unions), and p ~ E: the union p - q is redundant.  So p needs
+    // unions only where some uncovered q sits at a break of its ring column
+    // (bit k of the column's mask for q at p's z, bit k - 1 one plane below;
+    // bit 0 always set, as ring voxel 0 is not joined to the plane below).
+    uint32_t rcand = ~0u;  // bit k: a border run member at k may need unions
+    if constexpr (RBREAK) {
+      if (!covers_all) {
+        constexpr uint32_t PX = packed_offsets<true>(0), PY = packed_offsets<true>(1), PZ = packed_offsets<true>(2);
+        uint32_t un = ALLN & ~(cover_from<NB>(0, bdc) | 1u);
+        rcand = 0;
+        while (un) {
+          const int i = __builtin_ctz(un);
+          un &= un - 1u;
+          const int cbx = lx + (int)((PX >> (2 * i)) & 3u);  // box x: lx + 1 + dx
+          const int cby = ly0 + (int)((PY >> (2 * i)) & 3u);
+          const bool below = ((PZ >> (2 * i)) & 3u) == 0u;  // dz = -1
+          const int r = cby == 0 ? cbx : cby == G::BY - 1 ? G::BX + cbx : 2 * G::BX + 2 * (cby - 1) + (cbx != 0);
+          const uint32_t m = L.h.rbreak[r];
+          rcand |= below ? (m << 1) : m;
+        }
+      }
+    }
 #pragma unroll
     for (int k = 0; k < PT; k++) {
       const T cur = col[k];
@@ -797,6 +836,9 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       bool q = !((runs >> k) & 1u) ||
                !column_done<T, MODE, NB, QK>(cur, k > 0 ? col[k - 1] : pv, k > 0 ? covers_all : covers_all0,
                                              delta, gmin, gmax);
+      if constexpr (RBREAK) {
+        if (q && k > 0 && ((runs >> k) & 1u)) q = ((rcand >> k) & 1u) != 0u;
+      }
       if constexpr (NB == 6 && MODE != M_CONT) {
         // a 6-connected run member is queued only for a union it still needs
         if (q && ((runs >> k) & 1u)) {
@@ -1037,10 +1079,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     const uint32_t ex = incl - c0 - c1;
     if (w0 < RW) L.late.rprefix[w0] = ex;
     if (w0 + 1 < RW) L.late.rprefix[w0 + 1] = ex + c0;
-    if (lane == 63) L.nroots = incl;
+    if (lane == 63) L.late.nroots = incl;
   }
   __syncthreads();
-  const uint32_t R = L.nroots;
+  const uint32_t R = L.late.nroots;
   auto rank = [&](uint32_t r) -> uint32_t {
     return L.late.rprefix[r >> 5] + (uint32_t)__popc(L.w.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
   };
