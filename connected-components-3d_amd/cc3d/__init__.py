@@ -9,6 +9,8 @@ from ._batch import LabelBatch, connected_components_batch
 from ._core import DimensionError, connected_components, estimate_provisional_labels
 from ._follow import dust, largest_k, statistics
 from ._graphs import color_connectivity_graph, voxel_connectivity_graph
+from ._regions import contacts, draw, each, region_graph, runs
+from . import fastcc3d
 from ._serve import LabelServer, Request
 
 __version__ = "4.0.0+mi355x.0.1"
@@ -21,6 +23,9 @@ __all__ = [
   "largest_k",
   "voxel_connectivity_graph",
   "color_connectivity_graph",
+  "contacts",
+  "region_graph",
+  "each",
   "DimensionError",
   "connected_components_batch",
   "LabelBatch",

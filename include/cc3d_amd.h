@@ -248,6 +248,48 @@ int cc3d_amd_color_connectivity_graph(const void* vcg, int vcg_size, int64_t sx,
                                       int connectivity, uint32_t* out, uint64_t* N, void* stream);
 const char* cc3d_amd_color_last_error(void);
 
+/* Replaces extract_region_graph<T> (cc3d/cc3d_graphs.hpp:430-468, with
+ * extract_region_graph_2d/_3d at :305-428; bound at cc3d/fastcc3d.pyx:1198-1252
+ * as contacts / region_graph): the label pairs in contact and, per pair, the
+ * float32 sum the reference accumulates (face areas from wx, wy, wz, or
+ * neighbour counts when surface_area is 0), bit for bit.  labels: device, x
+ * fastest; dtype 0 u8, 1 u16, 2 u32, 3 u64, 6 i8, 7 i16, 8 i32, 9 i64 (pairs
+ * are ordered by the type's own comparison).  connectivity 6/18/26, or 4/8
+ * with sz == 1 (status 1, "z thickness must be 1 for 2d region graph
+ * extraction.").  On success *result holds n_pairs pairs on the device. */
+typedef struct cc3d_amd_contacts cc3d_amd_contacts;
+int cc3d_amd_contacts_run(const void* labels, int dtype, int64_t sx, int64_t sy, int64_t sz,
+                          int connectivity, int surface_area, float wx, float wy, float wz,
+                          cc3d_amd_contacts** result, uint64_t* n_pairs, void* stream);
+/* Copies the pairs to host arrays of n_pairs entries: lo < hi as 64-bit
+ * words (signed labels sign-extended), and the sums. */
+int cc3d_amd_contacts_fetch(cc3d_amd_contacts* result, uint64_t* lo, uint64_t* hi, float* value);
+void cc3d_amd_contacts_free(cc3d_amd_contacts* result);
+const char* cc3d_amd_contacts_last_error(void);
+
+/* Replaces extract_runs<T> (cc3d/cc3d_graphs.hpp:470-501, bound at
+ * cc3d/fastcc3d.pyx:1254-1277 as runs): the maximal runs of equal values of
+ * the flat array labels (device, label_size 1/2/4/8 bytes, unsigned), as
+ * (label, start, end) sorted by label, each label's runs in position order.
+ * Label-0 runs are included (first); the reference leaves them out except
+ * for a one-voxel array.  n < 2^32 - 1. */
+typedef struct cc3d_amd_runs cc3d_amd_runs;
+int cc3d_amd_extract_runs(const void* labels, int label_size, uint64_t n, cc3d_amd_runs** result, uint64_t* n_runs,
+                          void* stream);
+/* Device pointers to the n_runs (label, start, end) entries, valid until free. */
+int cc3d_amd_runs_device(cc3d_amd_runs* result, const uint64_t** label, const uint64_t** start,
+                         const uint64_t** end);
+int cc3d_amd_runs_fetch(cc3d_amd_runs* result, uint64_t* label, uint64_t* start, uint64_t* end);
+void cc3d_amd_runs_free(cc3d_amd_runs* result);
+/* Replaces set_run_voxels<T> (cc3d/cc3d_graphs.hpp:503-523, bound at
+ * cc3d/fastcc3d.pyx:1279-1308 as draw): image[start[k] .. end[k]) = value for
+ * every run k (image: device, image_size 1/2/4/8 bytes, n elements; start,
+ * end: device).  A run with end > n or start >= end stops the painting there:
+ * the runs before it are painted and status 1 ("Invalid run.") returned. */
+int cc3d_amd_draw(void* image, int image_size, uint64_t n, uint64_t value, const uint64_t* start,
+                  const uint64_t* end, uint64_t n_runs, void* stream);
+const char* cc3d_amd_runs_last_error(void);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  No
  * reference counterpart (measurement support).  on: 0 off, 1 every kernel,
  * 2 only the local labelling kernel.  collect() waits for the events and

@@ -700,3 +700,137 @@ uint64_t oracle_color_vcg(const void* vcg, int vcg_size, int64_t sx, int64_t sy,
   free(par);
   return N;
 }
+
+/*
+ * oracle_contacts: cc3d.contacts restated (cc3d/cc3d_graphs.hpp:238-468,
+ * bound at cc3d/fastcc3d.pyx:1198-1252).  The volume is walked in raster
+ * order (x fastest); for each nonzero voxel the first connectivity / 2
+ * backward offsets are visited, formed as the reference forms them:
+ *   3D: -x, -y, -z; the in-plane and z diagonals (only when both of their
+ *       directions exist); the four z corners, tested on y and z only, so on
+ *       the x borders they fall back to the (0, -1, -1) / (0, +1, -1) edge;
+ *   2D: -x, -y, (-x) + (-y), (+x) + (-y), each direction present or 0, with
+ *       no test of both -- so on the borders a diagonal becomes a face
+ *       neighbour again, or on row 0 the forward neighbour (x + 1).
+ * A neighbour holding another nonzero label adds the offset's area to the
+ * float32 sum of the pair (smaller label first; signed types compare
+ * signed): 3D faces wy*wz, wx*wz, wx*wy, 2D faces wy, wx, other offsets 0;
+ * with surface_area 0 every offset adds 1.  A pair first seen through an
+ * offset of area 0 still exists with sum 0.  lab: labels widened to 64 bits
+ * (sign-extended when is_signed).  Returns the number of pairs (written to
+ * lo/hi/val, at most cap), or -1 when cap is too small.
+ */
+typedef struct {
+  uint64_t lo, hi;
+  float v;
+  int used;
+} PairCell;
+
+static int pair_less(uint64_t a, uint64_t b, int is_signed) {
+  return is_signed ? (int64_t)a < (int64_t)b : a < b;
+}
+
+int64_t oracle_contacts(const uint64_t* lab, int is_signed, int64_t sx, int64_t sy, int64_t sz, int connectivity,
+                        int surface_area, float wx, float wy, float wz, uint64_t* lo, uint64_t* hi, float* val,
+                        int64_t cap) {
+  const int d2 = connectivity == 4 || connectivity == 8;
+  float areas[13];
+  if (d2) {
+    areas[0] = wy; areas[1] = wx; areas[2] = 0.0f; areas[3] = 0.0f;
+  } else {
+    for (int i = 3; i < 13; i++) areas[i] = 0.0f;
+    areas[0] = wy * wz; areas[1] = wx * wz; areas[2] = wx * wy;
+  }
+  if (!surface_area)
+    for (int i = 0; i < 13; i++) areas[i] = 1.0f;
+  int64_t hcap = 1024;
+  while (hcap < 4 * cap) hcap <<= 1;
+  PairCell* h = (PairCell*)calloc((size_t)hcap, sizeof(PairCell));
+  int64_t n = 0;
+  const int64_t sxy = sx * sy;
+  const int nn = connectivity / 2;
+  for (int64_t z = 0; z < sz; z++)
+    for (int64_t y = 0; y < sy; y++)
+      for (int64_t x = 0; x < sx; x++) {
+        const int64_t loc = x + sx * y + sxy * z;
+        const uint64_t cur = lab[loc];
+        if (cur == 0) continue;
+        int64_t nb[13];
+        const int64_t px = x < sx - 1, mx = -(int64_t)(x > 0);
+        const int64_t py = y < sy - 1 ? sx : 0, my = y > 0 ? -sx : 0, mz = z > 0 ? -sxy : 0;
+        if (d2) {
+          nb[0] = mx; nb[1] = my; nb[2] = mx + my; nb[3] = px + my;
+        } else {
+          nb[0] = mx; nb[1] = my; nb[2] = mz;
+          nb[3] = (mx && my) ? mx + my : 0;
+          nb[4] = (px && my) ? px + my : 0;
+          nb[5] = (mx && mz) ? mx + mz : 0;
+          nb[6] = (px && mz) ? px + mz : 0;
+          nb[7] = (my && mz) ? my + mz : 0;
+          nb[8] = (py && mz) ? py + mz : 0;
+          nb[9] = (my && mz) ? mx + my + mz : 0;
+          nb[10] = (my && mz) ? px + my + mz : 0;
+          nb[11] = (py && mz) ? mx + py + mz : 0;
+          nb[12] = (py && mz) ? px + py + mz : 0;
+        }
+        for (int i = 0; i < nn; i++) {
+          const uint64_t v = lab[loc + nb[i]];
+          if (v == 0 || v == cur) continue;
+          const uint64_t a = pair_less(cur, v, is_signed) ? cur : v, b = pair_less(cur, v, is_signed) ? v : cur;
+          uint64_t k = (a * 0x9E3779B97F4A7C15ull) ^ (b + 0x7F4A7C15ull + (a << 6) + (a >> 2));
+          int64_t s = (int64_t)(k & (uint64_t)(hcap - 1));
+          while (h[s].used && !(h[s].lo == a && h[s].hi == b)) s = (s + 1) & (hcap - 1);
+          if (!h[s].used) {
+            if (n >= cap) {
+              free(h);
+              return -1;
+            }
+            h[s].used = 1;
+            h[s].lo = a;
+            h[s].hi = b;
+            h[s].v = 0.0f;
+            n++;
+          }
+          h[s].v += areas[i];
+        }
+      }
+  int64_t m = 0;
+  for (int64_t s = 0; s < hcap; s++)
+    if (h[s].used) {
+      lo[m] = h[s].lo;
+      hi[m] = h[s].hi;
+      val[m] = h[s].v;
+      m++;
+    }
+  free(h);
+  return m;
+}
+
+/*
+ * oracle_runs: cc3d's runs() restated (cc3d/cc3d_graphs.hpp:470-501): the
+ * maximal runs of equal values of the flat array, as (label, start, end),
+ * nonzero labels only -- except that a one-voxel array gives its single run
+ * whatever its value.  Runs come out in position order.  Returns the count.
+ */
+int64_t oracle_runs(const uint64_t* lab, int64_t n, uint64_t* label, uint64_t* start, uint64_t* end) {
+  if (n == 0) return 0;
+  if (n == 1) {
+    label[0] = lab[0]; start[0] = 0; end[0] = 1;
+    return 1;
+  }
+  int64_t m = 0, s = 0;
+  uint64_t cur = lab[0];
+  for (int64_t i = 1; i < n; i++) {
+    if (lab[i] != cur) {
+      if (cur != 0) {
+        label[m] = cur; start[m] = (uint64_t)s; end[m] = (uint64_t)i; m++;
+      }
+      cur = lab[i];
+      s = i;
+    }
+  }
+  if (cur != 0) {
+    label[m] = cur; start[m] = (uint64_t)s; end[m] = (uint64_t)n; m++;
+  }
+  return m;
+}

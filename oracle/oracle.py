@@ -567,3 +567,156 @@ def color_connectivity_graph(vcg, connectivity=26, return_N=False, backend: str 
   while out.ndim > dims:
     out = out[..., 0]
   return (out, N) if return_N else out
+
+
+# ---------------------------------------------------------------------------
+# contacts / region_graph (cc3d/fastcc3d.pyx:1180-1252) and runs / draw /
+# each (cc3d/fastcc3d.pyx:1254-1372)
+# ---------------------------------------------------------------------------
+CT_CODE = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2, np.dtype(np.uint64): 3,
+           np.dtype(np.int8): 6, np.dtype(np.int16): 7, np.dtype(np.int32): 8, np.dtype(np.int64): 9,
+           np.dtype(bool): 0}
+
+
+def contacts(labels, connectivity=26, surface_area=True, anisotropy=(1, 1, 1), backend: str = "oracle"):
+  """cc3d.contacts restated (cc3d/fastcc3d.pyx:1198-1252): labels padded to 3
+  axes, anisotropy to 3 weights, connectivity checked, F order; the C
+  restatement (oracle_contacts) or the compiled reference computes the sums.
+  Returns {(lo, hi): float}."""
+  labels = np.asarray(labels)
+  while len(labels.shape) < 3:
+    labels = labels[..., np.newaxis]
+  anisotropy = tuple(anisotropy)
+  while len(anisotropy) < 3:
+    anisotropy = tuple(anisotropy) + (1,)
+  if connectivity not in (4, 8, 6, 18, 26):
+    raise ValueError(f"Only (2d) 4, 8, (3d) 6, 18, and 26 connectivities are supported. Got: {connectivity}")
+  if labels.dtype not in CT_CODE:
+    raise TypeError("No matching signature found")
+  labels = np.asfortranarray(labels)
+  sx, sy, sz = labels.shape
+  if labels.size == 0:
+    raise IndexError("Out of bounds on buffer access (axis %d)" % [i for i, s in enumerate(labels.shape) if s == 0][0])
+  code = CT_CODE[labels.dtype]
+  signed = labels.dtype.kind == "i"
+  flat = np.ravel(labels, order="F")
+  if labels.dtype == bool:
+    flat = flat.view(np.uint8)
+  wx, wy, wz = (float(np.float32(a)) for a in anisotropy)
+  cap = int(min(max(1024, 13 * flat.size), 1 << 26))
+  lo = np.zeros(cap, np.uint64)
+  hi = np.zeros(cap, np.uint64)
+  val = np.zeros(cap, np.float32)
+  if backend == "ref":
+    lib = _lib_ref()
+    lib.ref_contacts.restype = ctypes.c_int64
+    lib.ref_contacts.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                 ctypes.c_char_p, ctypes.c_int]
+    err = ctypes.create_string_buffer(256)
+    data = np.ascontiguousarray(flat)
+    n = lib.ref_contacts(data.ctypes.data, code, sx, sy, sz, int(connectivity), int(bool(surface_area)), wx, wy, wz,
+                         lo.ctypes.data, hi.ctypes.data, val.ctypes.data, cap, err, 256)
+    if n == -2:
+      raise RuntimeError(err.value.decode())
+  else:
+    if connectivity in (4, 8) and sz != 1:
+      raise RuntimeError("z thickness must be 1 for 2d region graph extraction.")
+    lib = _lib_oracle()
+    lib.oracle_contacts.restype = ctypes.c_int64
+    lib.oracle_contacts.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    wide = flat.astype(np.int64).view(np.uint64) if signed else flat.astype(np.uint64)
+    n = lib.oracle_contacts(wide.ctypes.data, int(signed), sx, sy, sz, int(connectivity), int(bool(surface_area)),
+                            wx, wy, wz, lo.ctypes.data, hi.ctypes.data, val.ctypes.data, cap)
+  if n < 0:
+    raise MemoryError("pair capacity exceeded")
+  lo, hi, val = lo[:n], hi[:n], val[:n]
+  if signed:
+    lo, hi = lo.view(np.int64), hi.view(np.int64)
+  return {(int(a), int(b)): float(v) for a, b, v in zip(lo.tolist(), hi.tolist(), val.tolist())}
+
+
+def region_graph(labels, connectivity=26, backend: str = "oracle"):
+  """cc3d.region_graph (cc3d/fastcc3d.pyx:1180-1196): the keys of contacts."""
+  return set(contacts(labels, connectivity=connectivity, backend=backend).keys())
+
+
+def _reshape_flat(arr: np.ndarray) -> np.ndarray:
+  """_reshape(arr, (arr.size,)) (cc3d/fastcc3d.pyx:129-161): a view in memory
+  order for F- or C-contiguous arrays, else a C-order copy."""
+  if arr.flags.f_contiguous:
+    return np.ravel(arr, order="F") if arr.ndim else arr.reshape(arr.size)
+  if arr.flags.c_contiguous:
+    return np.ravel(arr, order="C")
+  return arr.reshape((arr.size,))
+
+
+def runs(labels, backend: str = "oracle"):
+  """cc3d.fastcc3d.runs restated (cc3d/fastcc3d.pyx:1254-1277,
+  cc3d/cc3d_graphs.hpp:470-501): {label: [(start, end), ...]} over the
+  flat array in memory order."""
+  labels = np.asarray(labels)
+  flat = _reshape_flat(labels)
+  if flat.dtype not in (np.uint8, np.bool_, np.uint16, np.uint32, np.uint64):
+    raise TypeError("Unsupported type: " + str(flat.dtype))
+  if flat.dtype == np.bool_:
+    flat = flat.view(np.uint8)
+  n = flat.size
+  lab = np.zeros(max(n, 1), np.uint64)
+  st = np.zeros(max(n, 1), np.uint64)
+  en = np.zeros(max(n, 1), np.uint64)
+  if backend == "ref":
+    lib = _lib_ref()
+    lib.ref_runs.restype = ctypes.c_int64
+    lib.ref_runs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p, ctypes.c_int64]
+    data = np.ascontiguousarray(flat)
+    m = lib.ref_runs(data.ctypes.data if n else None, CT_CODE[data.dtype], n, lab.ctypes.data, st.ctypes.data,
+                     en.ctypes.data, max(n, 1))
+    out = {}
+    for a, s, e in zip(lab[:m].tolist(), st[:m].tolist(), en[:m].tolist()):
+      out.setdefault(a, []).append((s, e))
+    return out
+  lib = _lib_oracle()
+  lib.oracle_runs.restype = ctypes.c_int64
+  lib.oracle_runs.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+  wide = flat.astype(np.uint64)
+  m = lib.oracle_runs(wide.ctypes.data, n, lab.ctypes.data, st.ctypes.data, en.ctypes.data)
+  out = {}
+  for a, s, e in zip(lab[:m].tolist(), st[:m].tolist(), en[:m].tolist()):
+    out.setdefault(a, []).append((s, e))
+  return {k: out[k] for k in sorted(out)}
+
+
+def draw(label, rns, image):
+  """cc3d.fastcc3d.draw restated (cc3d/fastcc3d.pyx:1279-1308,
+  cc3d/cc3d_graphs.hpp:503-523): paints label (label != 0 for a bool image)
+  over each run of the flat view of image, checking each run as it comes
+  ("Invalid run.": end past the image, or start >= end); returns the flat view."""
+  flat = _reshape_flat(image)
+  if flat.dtype not in (np.bool_, np.uint8, np.uint16, np.uint32, np.uint64):
+    raise TypeError("Unsupported type: " + str(flat.dtype))
+  val = (label != 0) if flat.dtype == np.bool_ else label
+  for s, e in rns:
+    if e > flat.size or s >= e:
+      raise RuntimeError("Invalid run.")
+    flat[s:e] = val
+  return flat
+
+
+def each(labels, binary=False, in_place=False):
+  """cc3d.each restated (cc3d/fastcc3d.pyx:1310-1372): (label, image) per
+  nonzero label in label order."""
+  labels = np.asarray(labels)
+  all_runs = runs(labels)
+  order = "F" if labels.flags.f_contiguous else "C"
+  dtype = bool if binary else labels.dtype
+  for key, rns in all_runs.items():
+    if key == 0:
+      continue
+    img = np.zeros(labels.shape, dtype=dtype, order=order)
+    draw(key, rns, img)
+    yield key, img

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 
 #include "cc3d_continuous.hpp"
 #include "cc3d_graphs.hpp"
@@ -49,6 +50,42 @@ void run_out(const void* in, int64_t sx, int64_t sy, int64_t sz,
 }
 
 }  // namespace
+
+template <class T>
+static int64_t region_graph_of(const void* labels, int64_t sx, int64_t sy, int64_t sz, float wx, float wy, float wz,
+                               int connectivity, int surface_area, uint64_t* lo, uint64_t* hi, float* val,
+                               int64_t cap) {
+  const auto res = cc3d::extract_region_graph<T>(static_cast<T*>(const_cast<void*>(labels)), sx, sy, sz, wx, wy, wz,
+                                                 connectivity, surface_area != 0);
+  if ((int64_t)res.size() > cap) return -1;
+  int64_t m = 0;
+  for (const auto& kv : res) {
+    lo[m] = (uint64_t)(int64_t)kv.first.first;
+    hi[m] = (uint64_t)(int64_t)kv.first.second;
+    if (!std::is_signed<T>::value) {
+      lo[m] = (uint64_t)kv.first.first;
+      hi[m] = (uint64_t)kv.first.second;
+    }
+    val[m] = kv.second;
+    m++;
+  }
+  return m;
+}
+
+template <class T>
+static int64_t runs_of(const void* labels, uint64_t n, uint64_t* label, uint64_t* start, uint64_t* end, int64_t cap) {
+  const auto res = cc3d::extract_runs<T>(static_cast<T*>(const_cast<void*>(labels)), n);
+  int64_t m = 0;
+  for (const auto& kv : res)
+    for (const auto& r : kv.second) {
+      if (m >= cap) return -1;
+      label[m] = (uint64_t)kv.first;
+      start[m] = r.first;
+      end[m] = r.second;
+      m++;
+    }
+  return m;
+}
 
 extern "C" {
 
@@ -169,6 +206,47 @@ int ref_color_connectivity_graph(const void* vcg, int vcg_size, int64_t sx, int6
     return 1;
   }
   return 0;
+}
+
+// cc3d/cc3d_graphs.hpp:430-468 (extract_region_graph<T>) through
+// cc3d/fastcc3d.pyx:1230-1252: labels x fastest, dtype 0 u8, 1 u16, 2 u32,
+// 3 u64, 6 i8, 7 i16, 8 i32, 9 i64.  Pairs are written as 64-bit words
+// (signed labels sign-extended) with their sums, at most cap of them.
+// Returns the pair count, -1 when cap is too small, -2 on a C++ exception.
+int64_t ref_contacts(const void* labels, int dtype, int64_t sx, int64_t sy, int64_t sz, int connectivity,
+                     int surface_area, float wx, float wy, float wz, uint64_t* lo, uint64_t* hi, float* val,
+                     int64_t cap, char* errbuf, int errlen) {
+  try {
+    switch (dtype) {
+      case 0: return region_graph_of<uint8_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 1: return region_graph_of<uint16_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 2: return region_graph_of<uint32_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 3: return region_graph_of<uint64_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 6: return region_graph_of<int8_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 7: return region_graph_of<int16_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      case 8: return region_graph_of<int32_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+      default: return region_graph_of<int64_t>(labels, sx, sy, sz, wx, wy, wz, connectivity, surface_area, lo, hi, val, cap);
+    }
+  } catch (const std::exception& e) {
+    if (errbuf && errlen > 0) {
+      std::strncpy(errbuf, e.what(), errlen - 1);
+      errbuf[errlen - 1] = 0;
+    }
+    return -2;
+  }
+}
+
+// cc3d/cc3d_graphs.hpp:470-501 (extract_runs<T>) through
+// cc3d/fastcc3d.pyx:1254-1277: the runs of each label in label order, each
+// label's runs in position order, flattened as (label, start, end).
+int64_t ref_runs(const void* labels, int dtype, uint64_t n, uint64_t* label, uint64_t* start, uint64_t* end,
+                 int64_t cap) {
+  switch (dtype) {
+    case 0: return runs_of<uint8_t>(labels, n, label, start, end, cap);
+    case 1: return runs_of<uint16_t>(labels, n, label, start, end, cap);
+    case 2: return runs_of<uint32_t>(labels, n, label, start, end, cap);
+    default: return runs_of<uint64_t>(labels, n, label, start, end, cap);
+  }
 }
 
 }  // extern "C"

@@ -410,3 +410,70 @@ def test_color_graph_checks_catch_planted_mistake(monkeypatch):
   flipped[0, 0] = 0b01  # only the +x side set: no join
   b, m = oracle.color_connectivity_graph(flipped, connectivity=4, return_N=True)
   assert m == 12
+
+
+def test_oracle_contacts_known_answers():
+  """contacts / region_graph: the reference's own cases (automated_test.py:689-877)."""
+  import region_known_answers as ka
+  ka.check_contacts(lambda v, **k: oracle.contacts(v, **k), lambda v, **k: oracle.region_graph(v, **k))
+  if oracle.have_ref():
+    ka.check_contacts(lambda v, **k: oracle.contacts(v, backend="ref", **k),
+                      lambda v, **k: oracle.region_graph(v, backend="ref", **k))
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built")
+def test_oracle_contacts_match_compiled_reference():
+  """The restatement against extract_region_graph compiled from its header,
+  float sums compared bit for bit (non-dyadic weights included)."""
+  import region_known_answers as ka
+  rng = np.random.default_rng(17)
+  for lab, conn, sa, an in ka.random_label_volumes(rng, 200):
+    a = oracle.contacts(lab, connectivity=conn, surface_area=sa, anisotropy=an)
+    b = oracle.contacts(lab, connectivity=conn, surface_area=sa, anisotropy=an, backend="ref")
+    assert a == b, (lab.shape, lab.dtype, conn, sa, an)
+
+
+def test_contacts_checks_catch_planted_mistakes():
+  """What the checks above must catch: ignoring the reference's doubled
+  corner offsets on the x borders changes neighbour counts; summing the
+  areas in another order changes non-dyadic sums."""
+  lab = np.zeros((3, 3, 3), dtype=np.uint32)
+  lab[0, 1, 1] = 1
+  lab[0, 0, 0] = 2  # (0, -1, -1) from (0, 1, 1): the edge, also named by a corner offset at x = 0
+  res = oracle.contacts(lab, connectivity=26, surface_area=False)
+  assert res[(1, 2)] == 2  # counted twice, as the reference does
+  res = oracle.contacts(lab, connectivity=18, surface_area=False)
+  assert res[(1, 2)] == 1
+  # order matters for non-dyadic sums: 0.1f added 10 times is not 10 * 0.1f
+  acc = np.float32(0)
+  for _ in range(10):
+    acc = np.float32(acc + np.float32(0.1))
+  assert acc != np.float32(10 * np.float32(0.1))
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built")
+def test_oracle_runs_match_compiled_reference():
+  rng = np.random.default_rng(3)
+  for i in range(200):
+    n = int(rng.integers(0, 60))
+    dt = [np.uint8, np.uint16, np.uint32, np.uint64, bool][i % 5]
+    lab = rng.integers(0, 3, size=n).astype(dt)
+    a, b = oracle.runs(lab), oracle.runs(lab, backend="ref")
+    assert a == b and list(a) == list(b)
+  assert oracle.runs(np.zeros(1, np.uint8)) == {0: [(0, 1)]}  # the one-voxel special case
+
+
+def test_oracle_each_known_answer():
+  """cc3d.each (automated_test.py:660-676) on the restatement."""
+  import region_known_answers as ka
+
+  def each(labels, binary=False, in_place=False):
+    class It:
+      def __len__(self):
+        r = oracle.runs(labels)
+        return len(r) - int(0 in r)
+
+      def __iter__(self):
+        return oracle.each(labels, binary=binary, in_place=in_place)
+    return It()
+  ka.check_each(each, side=24)
