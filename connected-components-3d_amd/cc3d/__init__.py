@@ -10,6 +10,7 @@ from ._core import DimensionError, connected_components, estimate_provisional_la
 from ._follow import dust, largest_k, statistics
 from ._graphs import color_connectivity_graph, voxel_connectivity_graph
 from ._regions import contacts, draw, each, region_graph, runs
+from ._stack import connected_components_stack
 from . import fastcc3d
 from ._serve import LabelServer, Request
 
@@ -17,6 +18,7 @@ __version__ = "4.0.0+mi355x.0.1"
 
 __all__ = [
   "connected_components",
+  "connected_components_stack",
   "estimate_provisional_labels",
   "statistics",
   "dust",

@@ -372,6 +372,9 @@ def connected_components(
 
 def _label_one(pl):
   _require_gpu()
+  from . import _stack
+  if _stack.streamable(pl):
+    return _stack.label_streamed(pl)
   torch = _torch()
   buf = pl.vol.device_buffer(pl.order)
   sx, sy, sz, connectivity = pl.sx, pl.sy, pl.sz, pl.connectivity

@@ -290,6 +290,38 @@ int cc3d_amd_draw(void* image, int image_size, uint64_t n, uint64_t value, const
                   const uint64_t* end, uint64_t n_runs, void* stream);
 const char* cc3d_amd_runs_last_error(void);
 
+/* Streaming a volume in z-slabs (cc3d/__init__.py:353-501,
+ * connected_components_stack; no single-call counterpart: the reference
+ * labels each slab with connected_components and merges seams in Python).
+ * Slab k's labels 1..N_k become provisional ids offset_k + label; a 64-bit
+ * forest over the ids (entry p = parent, union by the smaller id) joins the
+ * seams; the final label of id p is 1 + the rank of its root.
+ *
+ * stack_seam: unions across the seam between slab k-1's last plane
+ * (prev_values, prev_labels; device, sx*sy, x fastest) and slab k's first
+ * plane, for 6/18/26-connectivity; labels are the slabs' own (uint32), the
+ * offsets turn them into ids.  dtype as the labelling call (0 u8 .. 5 f64). */
+int cc3d_amd_stack_seam(const void* prev_values, const uint32_t* prev_labels, const void* values,
+                        const uint32_t* labels, int dtype, int64_t sx, int64_t sy, uint64_t prev_offset,
+                        uint64_t offset, int connectivity, int binary_image, uint64_t* forest, void* stream);
+/* Adds to *count (device) the foreground voxels of the first plane that take
+ * a new provisional label in their slab's own pass but not in a pass over
+ * the whole volume (they link to the previous slab's last plane). */
+int cc3d_amd_stack_seam_provisional(const void* prev_values, const void* values, int dtype, int64_t sx,
+                                    int64_t sy, int connectivity, int binary_image, uint64_t* count,
+                                    void* stream);
+/* forest[p] = p for first <= p < last (device). */
+int cc3d_amd_forest_init(uint64_t* forest, uint64_t first, uint64_t last, void* stream);
+/* lut[p] (device, n_ids + 1 entries) = final label of id p (lut[0] = 0);
+ * *N = number of components (host).  Waits for the stream. */
+int cc3d_amd_forest_labels(const uint64_t* forest, uint64_t n_ids, uint64_t* lut, uint64_t* N, void* stream);
+/* out[i] = labels[i] ? labels[i] + offset : 0 (uint32 in, uint64 out, device). */
+int cc3d_amd_offset_labels(const uint32_t* labels, uint64_t n, uint64_t offset, uint64_t* out, void* stream);
+/* out[i] = lut[ids[i]] in the output dtype (1 u16, 2 u32, 3 u64; device). */
+int cc3d_amd_map_labels(const uint64_t* ids, uint64_t n, const uint64_t* lut, void* out, int out_dtype,
+                        void* stream);
+const char* cc3d_amd_stack_last_error(void);
+
 /* Per-kernel timing with HIP events recorded on the launch stream.  No
  * reference counterpart (measurement support).  on: 0 off, 1 every kernel,
  * 2 only the local labelling kernel.  collect() waits for the events and
