@@ -4,10 +4,12 @@
 //   k_init       reset counters, clear the root bitmask
 //   [k_minmax]   continuous 2D-8 only: global min/max (cc3d/cc3d_continuous.hpp:298-303)
 //   k_local      per tile: LDS union-find, local roots, J tables, requests
-//   k_requests   per tile: cross-tile unions on the global key forest GP
+//   k_requests   per tile: cross-tile unions on the global forest GP over local
+//                roots (entry tile_root0 + r per local root r), each tree rooted at
+//                its smallest key
 //   [k_periodic] wrap-around unions (cc3d/cc3d.hpp:1048-1073, :1265-1277, :1377-1418)
-//   k_roots      per local root: find its global root, store it in RL; global roots
-//                set a bit at their key
+//   k_roots      per local root: find its global root, store the root's key in RL;
+//                global roots set a bit at their key
 //   k_scan_*     exclusive popcount prefix over the bitmask -> N
 //   --- stage 2 ---
 //   k_finalize   per local root: final label = 1 + rank of its global root's key;
@@ -190,16 +192,26 @@ __device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
   return g;
 }
 
-// Key of the local root of voxel (gx, gy, gz); 0xFFFFFFFF for background.
+// Forest entry of the local root of voxel (gx, gy, gz); 0xFFFFFFFF for background.
 template <class G>
-__device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int64_t gx, int64_t gy, int64_t gz) {
+__device__ inline uint32_t lroot_index(const GlobalParams& p, const VolGeo& v, int64_t gx, int64_t gy, int64_t gz) {
   const int64_t tx = gx / G::TX, ty = gy / G::TY, tz = G::IS3D ? gz / G::TZ : 0;
   const int64_t slot = v.tile0 + tx + v.ntx * (ty + v.nty * tz);
   const int ji = G::jidx((int)(gx - tx * G::TX), (int)(gy - ty * G::TY), (int)(gz - tz * G::TZ));
   const uint32_t R = p.TR[slot];
   const uint32_t j = R <= 255 ? p.J8[slot * G::TV + ji] : p.J16[slot * G::TV + ji];
   if (j == 0) return 0xFFFFFFFFu;
-  return p.RL[slot * G::TV + j - 1];
+  return tile_root0<G>(v.sx, v.sy, v.sz, tx, ty, tz, v.key0) + j - 1;
+}
+
+// First forest entry of tile `slot` of volume v.
+template <class G>
+__device__ inline uint32_t slot_root0(const VolGeo& v, int64_t slot) {
+  const int64_t tile = slot - v.tile0;
+  const int64_t tz = tile / (v.ntx * v.nty);
+  const int64_t tr = tile - tz * v.ntx * v.nty;
+  const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
+  return tile_root0<G>(v.sx, v.sy, v.sz, tx, ty, tz, v.key0);
 }
 
 // Lanes that share one tile in the per-tile global kernels: a tile holds a few
@@ -207,7 +219,9 @@ __device__ inline uint32_t lroot_key(const GlobalParams& p, const VolGeo& v, int
 // lanes idle.  The label pass does less work per root and runs best wider.
 constexpr int TILE_LANES = 8;
 
-// Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests.
+// Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests,
+// each (local root r, halo index h): the forest entry of r and that of the
+// local root owning halo voxel h in its own tile.
 template <class G, int QK>
 __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
   const int lane = threadIdx.x & 63, sub = lane % TILE_LANES;
@@ -222,17 +236,15 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
     const int64_t tr = tile - tz * v.ntx * v.nty;
     const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
     const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
+    const uint32_t root0 = tile_root0<G>(v.sx, v.sy, v.sz, tx, ty, tz, v.key0);
     for (uint32_t i = sub; i < nq; i += TILE_LANES) {
       const uint32_t rq = p.REQ[slot * G::HALO + i];
-      const int rn = rq >> 16, h = rq & 0xFFFF;
+      const uint32_t a = root0 + (rq >> 16);
       int bx, by, bz;
-      G::template nid_box<QK>(rn, bx, by, bz);  // the local root: a tile or a halo voxel
-      const uint32_t a =
-        gkey<QK>(x0 + bx - 1, y0 + by - 1, G::IS3D ? z0 + bz - 1 : 0, v.sx, v.sy, v.osx) + v.key0;
-      G::halo_coords(h, bx, by, bz);
+      G::halo_coords(rq & 0xFFFF, bx, by, bz);
       const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
-      const uint32_t b = lroot_key<G>(p, v, gx, gy, gz);
-      if (b != 0xFFFFFFFFu) gunion(p.GP, a, b);
+      const uint32_t b = lroot_index<G>(p, v, gx, gy, gz);
+      if (b != 0xFFFFFFFFu) kunion(p.GP, p.RL, a, b);
     }
   }
 }
@@ -339,24 +351,32 @@ __global__ __launch_bounds__(WG) void k_periodic(GlobalParams p, const T* in, in
     const bool link = MODE == M_BINARY ? fg(vb) : (va == vb);
     if (!link) continue;
     const VolGeo v = vol_geo(p, 0);
-    const uint32_t ka = lroot_key<G>(p, v, ax, ay, az), kb = lroot_key<G>(p, v, bx, by, bz);
-    if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) gunion(p.GP, ka, kb);
+    const uint32_t ka = lroot_index<G>(p, v, ax, ay, az), kb = lroot_index<G>(p, v, bx, by, bz);
+    if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) kunion(p.GP, p.RL, ka, kb);
   }
 }
 
-// Global roots: compress every local root onto its root; roots mark their key.
+// Global roots: every local root finds its global root (the component's
+// smallest key); roots mark their key, and RL takes the root's key.  A root's
+// own entry is never rewritten, so the keys read stay valid.
 template <class G>
 __global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
   const int sub = threadIdx.x % TILE_LANES;
   const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
   const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
-  for (int64_t tile = group; tile < p.ntiles; tile += ngroups) {
-    const uint32_t R = p.TR[tile];
+  for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
+    const uint32_t R = p.TR[slot];
+    if (R == 0) continue;
+    const uint32_t root0 = slot_root0<G>(vol_geo(p, slot), slot);
     for (uint32_t j = sub; j < R; j += TILE_LANES) {
-      const uint32_t g = p.RL[tile * G::TV + j];
-      const uint32_t r = gfind_settled(p.GP, g);
-      if (r == g) atomicOr(p.bits + (g >> 5), 1u << (g & 31));
-      p.RL[tile * G::TV + j] = r;  // from here on RL holds each local root's global root
+      const uint32_t c = root0 + j;
+      const uint32_t r = kfind_settled(p.GP, c);
+      if (r == c) {
+        const uint32_t key = p.RL[c];
+        atomicOr(p.bits + (key >> 5), 1u << (key & 31));
+      } else {
+        p.RL[c] = p.RL[r];  // from here on RL holds each local root's global root key
+      }
     }
   }
 }
@@ -483,7 +503,7 @@ __device__ inline uint32_t root_label(const GlobalParams& p, uint32_t r, uint32_
 // and the tile's label table: entry 0 background, entry 1 + i the label of
 // local root i.
 template <class G>
-__device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t base, uint32_t (&j)[8],
+__device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t root0, uint32_t base, uint32_t (&j)[8],
                                 uint32_t* lab) {
   static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
   const int t = threadIdx.x;
@@ -498,7 +518,7 @@ __device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t ba
 #pragma unroll
     for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
-  const uint32_t* __restrict__ LT = p.RL + slot * G::TV;
+  const uint32_t* __restrict__ LT = p.RL + root0;
   lab[0] = 0u;
   for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
 }
@@ -536,7 +556,7 @@ __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict
 #pragma unroll
   for (int n = 0; n < FIN_TILES; n++) {
     if (first + n >= p.ntiles) break;
-    fin_load<G>(p, first + n, 0u, j[n], lab[n]);
+    fin_load<G>(p, first + n, slot_root0<G>(v, first + n), 0u, j[n], lab[n]);
   }
   __syncthreads();
 #pragma unroll
@@ -561,7 +581,8 @@ __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
     const int64_t slot = first + n;
     if (slot >= p.ntiles) break;
     vi[n] = (int)p.vix[slot];
-    fin_load<G>(p, slot, p.prefix[p.vols[vi[n]].key0 >> 5], j[n], lab[n]);
+    fin_load<G>(p, slot, slot_root0<G>(batch_geo(p.vols[vi[n]]), slot), p.prefix[p.vols[vi[n]].key0 >> 5], j[n],
+                lab[n]);
   }
   __syncthreads();
 #pragma unroll
@@ -612,7 +633,7 @@ __device__ inline void rfin_tile(const GlobalParams& p, const VolGeo& v, int64_t
 #pragma unroll
     for (int k = 0; k < RFIN_PT; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
-  const uint32_t* __restrict__ LT = p.RL + slot * GeoR::TV;
+  const uint32_t* __restrict__ LT = p.RL + slot_root0<GeoR>(v, slot);
   if (t == 0) lab[0] = 0u;
   for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
   __syncthreads();
@@ -1174,7 +1195,7 @@ static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connect
   };
   const size_t slots = (size_t)g.ntiles * g.TV;
   g.oGP = take((size_t)g.K * 4);
-  g.oRL = take(slots * 4);
+  g.oRL = take((size_t)g.K * 4);
   g.oTR = take((size_t)g.ntiles * 4);
   g.oTQ = take((size_t)g.ntiles * 4);
   g.oTC = take((size_t)g.ntiles * sizeof(TileCnt));
@@ -1226,7 +1247,7 @@ struct cc3d_amd_batch {
   bool own_ws = false;
   size_t total = 0;
   int64_t ntiles = 0, nkeys = 0, nwords = 0, nblk = 0;
-  size_t oVOL = 0, oGP = 0, oBITS = 0, oPRE = 0, oBSUM = 0, oCTR = 0;
+  size_t oVOL = 0, oGP = 0, oRL = 0, oBITS = 0, oPRE = 0, oBSUM = 0, oCTR = 0;
   BatchVol* h_tab = nullptr;  // pinned host copy of the volume table
   int64_t* h_t0 = nullptr;    // pinned: first tile of each volume, per tile group (n + 2)
   Counters* h_ctr = nullptr;  // pinned: entry 0 whole batch, 1 + k table entry k
@@ -1244,7 +1265,7 @@ struct cc3d_amd_batch {
     bool rows = false;
     int v0 = 0, v1 = 0;  // table entries
     int64_t tiles = 0;
-    size_t oT0 = 0, oVIX = 0, oRL = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0;
+    size_t oT0 = 0, oVIX = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0;
     GlobalParams gp{};
     LocalParams lp{};
   };
@@ -1879,6 +1900,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   };
   b.oVOL = take((size_t)n * sizeof(BatchVol));
   b.oGP = take((size_t)b.nkeys * 4);
+  b.oRL = take((size_t)b.nkeys * 4);
   b.oBITS = take((size_t)b.nwords * 4);
   b.oPRE = take((size_t)b.nwords * 4);
   b.oBSUM = take((size_t)(b.nblk + 1) * 4);
@@ -1888,7 +1910,6 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
     const size_t slots = (size_t)tg.tiles * TVg;
     tg.oT0 = take((size_t)(tg.v1 - tg.v0 + 1) * sizeof(int64_t));
     tg.oVIX = take((size_t)tg.tiles * 4);
-    tg.oRL = take(slots * 4);
     tg.oTR = take((size_t)tg.tiles * 4);
     tg.oTQ = take((size_t)tg.tiles * 4);
     tg.oTC = take((size_t)tg.tiles * sizeof(TileCnt));
@@ -1979,7 +2000,7 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     GlobalParams& q = tg.gp;
     q = gp;
     q.ntiles = tg.tiles;
-    q.RL = reinterpret_cast<uint32_t*>(ws + tg.oRL);
+    q.RL = reinterpret_cast<uint32_t*>(ws + b->oRL);
     q.TR = reinterpret_cast<uint32_t*>(ws + tg.oTR);
     q.J8 = reinterpret_cast<uint8_t*>(ws + tg.oJ8);
     q.J16 = reinterpret_cast<uint16_t*>(ws + tg.oJ16);
@@ -2091,6 +2112,7 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
       else hipLaunchKernelGGL((k_roots<Geo3>), dim3(blocks), dim3(WG), 0, s, tg.gp);
     }
   }
+
   {
     ProfScope ps_("k_scan", s);
     hipLaunchKernelGGL(k_scan_a, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);

@@ -217,6 +217,20 @@ struct TileGeo {
 // 3D: 32 x 8 x 8 voxels; 2D: 64 x 32 pixels; 256 threads.  (A 32 x 16 x 4
 // tile with 512 threads puts fewer columns on the tile border but holds
 // fewer tiles per CU in flight: measured slower.)
+// Index of a tile's first local root in the per-volume root arrays (global
+// forest GP and root keys RL): tiles in raster order, each taking as many
+// entries as it holds voxels of the volume (a tile has at most one local root
+// per voxel), so the entries of a volume are exactly [key0, key0 + V) and a
+// tile's roots sit together.
+template <class G>
+__host__ __device__ inline uint32_t tile_root0(int64_t sx, int64_t sy, int64_t sz, int64_t tx, int64_t ty, int64_t tz,
+                                               uint32_t key0) {
+  const int64_t z0 = tz * G::TZ, y0 = ty * G::TY, x0 = tx * G::TX;
+  const int64_t wz = G::IS3D ? (sz - z0 < G::TZ ? sz - z0 : G::TZ) : 1;
+  const int64_t wy = sy - y0 < G::TY ? sy - y0 : G::TY;
+  return key0 + (uint32_t)(sx * sy * z0 + wz * sx * y0 + wz * wy * x0);
+}
+
 using Geo3 = TileGeo<32, 8, 8, true, 256>;
 using Geo2 = TileGeo<64, 32, 1, false, 256>;
 
@@ -331,6 +345,52 @@ __device__ inline uint32_t lunion(uint16_t* par, uint32_t a, uint32_t b) {
     if (old == a) return b;
     a = old;
   }
+}
+
+// Global union-find over local roots (entries in tile order, RL[i] = key of
+// entry i, read-only while unions run): a root is linked under the root with
+// the smaller (key, entry) pair, so every tree is rooted at its component's
+// smallest key.  (Two entries can share a key: a halo voxel that is a local
+// root of a neighbouring tile and of its own tile.)  Links and path halving
+// are compare-and-swaps on the entry's parent, so a concurrent link is never
+// overwritten; (key, entry) only decreases along a path, so no cycle forms.
+__device__ inline uint32_t kfind(uint32_t* GP, uint32_t a) {
+  uint32_t p = __hip_atomic_load(GP + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (p != a) {
+    const uint32_t gp = __hip_atomic_load(GP + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (gp == p) return p;
+    atomicCAS(GP + a, p, gp);  // halving; loses only to another halving or to nothing
+    a = gp;
+    p = __hip_atomic_load(GP + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return a;
+}
+__device__ inline void kunion(uint32_t* GP, const uint32_t* RL, uint32_t a, uint32_t b) {
+  while (true) {
+    a = kfind(GP, a);
+    b = kfind(GP, b);
+    if (a == b) return;
+    const uint64_t ka = ((uint64_t)RL[a] << 32) | a, kb = ((uint64_t)RL[b] << 32) | b;
+    if (ka < kb) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(GP + a, a, b) == a) return;  // a was still a root: linked under b
+  }
+}
+// Find once every union is made: only compression remains, so any ancestor
+// stored into a node still leads to its root.
+__device__ inline uint32_t kfind_settled(uint32_t* GP, uint32_t a) {
+  uint32_t r = a;
+  for (uint32_t p = GP[r]; p != r; p = GP[r]) r = p;
+  while (a != r) {
+    const uint32_t p = GP[a];
+    if (p == r) break;
+    GP[a] = r;
+    a = p;
+  }
+  return r;
 }
 
 // Global union-find over keys.  Loads bypass the non-coherent L1 so retries see

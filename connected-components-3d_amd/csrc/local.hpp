@@ -38,13 +38,13 @@
 // common.hpp), so tiles never lean on each other's cross-tile links.
 //
 // Outputs per tile:
-//   RL[tile*TV + r]  global key of local root r
+//   RL[root0 + r]    global key of local root r (root0 = tile_root0)
 //   TR[tile]         number of local roots R
 //   J8/J16           per voxel (thread-major: thread t's column is entries
 //                    8t..8t+7): 0 (background) or 1 + index of its local root;
 //                    8-bit when R <= 255, else 16-bit
-//   GP[key]          initialised to key for every local root
-//   REQ/TQ           cross-tile union requests: (root position, halo index)
+//   GP[root0 + r]    initialised to itself for every local root
+//   REQ/TQ           cross-tile union requests: (root index r, halo index)
 //                    for each halo "piece" (halo voxels connected through links
 //                    that stay inside one neighbouring tile) touching a local
 //                    component
@@ -1086,7 +1086,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   auto rank = [&](uint32_t r) -> uint32_t {
     return L.late.rprefix[r >> 5] + (uint32_t)__popc(L.w.rbits[r >> 5] & ((1u << (r & 31)) - 1u));
   };
-  uint32_t* RLt = p.RL + slot * G::TV;
+  const uint32_t root0 = tile_root0<G>(sx, sy, sz, tx, ty, tz, keybase);
+  uint32_t* RLt = p.RL + root0;
   // Keys are below 2^32, so they can be formed in 32-bit arithmetic: the
   // wrap-around of the intermediate terms cancels.  Box position (bx, by, bz)
   // is voxel (x0 + bx - 1, y0 + by - 1, z0 + bz - 1).
@@ -1106,8 +1107,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         key = kbox0 + (uint32_t)bx + (uint32_t)sx * ((uint32_t)by + (uint32_t)sy * (uint32_t)bz) + keybase;
       else
         key = kbox0 + (uint32_t)sx * (uint32_t)sy + (uint32_t)bx + (uint32_t)sx * (uint32_t)by + keybase;
+      p.GP[root0 + rk] = root0 + rk;
       RLt[rk++] = key;
-      p.GP[key] = key;
     }
   }
 
@@ -1156,7 +1157,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     uint32_t* REQt = p.REQ + slot * G::HALO;
 #pragma unroll
     for (int i = 0; i < HV; i++)
-      if ((rq >> i) & 1u) REQt[q++] = (hroot[i] << 16) | hindex(i);
+      if ((rq >> i) & 1u) REQt[q++] = (rank(hroot[i]) << 16) | hindex(i);
   }
 
   // 9. counters

@@ -28,7 +28,7 @@
 // Halo pieces (halo nodes connected through links inside one neighbouring
 // tile) are unioned first, and their roots recorded; after the tile's own
 // unions each piece whose tree holds tile voxels makes one cross-tile request
-// (root node, piece node), resolved by the global kernels.  Outputs (RL, TR,
+// (root index, piece node), resolved by the global kernels.  Outputs (RL, TR,
 // J8/J16, GP, REQ/TQ, tcnt) have the same meaning as those of the voxel
 // kernel in local.hpp; J is row-major: entry x + 64 * (y + 8 * z).
 #pragma once
@@ -590,7 +590,8 @@ __device__ inline void rows_tile(const LocalParams& p, const T* in, int64_t sx, 
   //    cancel): box position (bx, by, bz) is voxel (x0 + bx - 1, ...).
   const uint32_t kbox0 =
     (uint32_t)(x0 - 1) + (uint32_t)sx * ((uint32_t)(y0 - 1) + (uint32_t)sy * (uint32_t)(z0 - 1)) + keybase;
-  uint32_t* RLt = p.RL + slot * G::TV;
+  const uint32_t root0 = tile_root0<G>(sx, sy, sz, tx, ty, tz, keybase);
+  uint32_t* RLt = p.RL + root0;
   if (t < RW) {
     uint32_t bits = L.rbits[t];
     uint32_t rk = L.u.rprefix[t];
@@ -600,8 +601,8 @@ __device__ inline void rows_tile(const LocalParams& p, const T* in, int64_t sx, 
       int bx, by, bz;
       G::halo_coords(32 * t + b, bx, by, bz);
       const uint32_t key = kbox0 + (uint32_t)bx + (uint32_t)sx * ((uint32_t)by + (uint32_t)sy * (uint32_t)bz);
+      p.GP[root0 + rk] = root0 + rk;
       RLt[rk++] = key;
-      p.GP[key] = key;
     }
   }
 #pragma unroll 2
@@ -621,7 +622,7 @@ __device__ inline void rows_tile(const LocalParams& p, const T* in, int64_t sx, 
     const uint32_t root = rfind_ro(par, h);
     if ((L.rbits[root >> 5] >> (root & 31)) & 1u) {
       const uint32_t k = atomicAdd(&L.nreq, 1u);
-      p.REQ[slot * G::HALO + k] = (root << 16) | h;
+      p.REQ[slot * G::HALO + k] = (rank(root) << 16) | h;
     }
   }
   // counters
