@@ -19,10 +19,14 @@ Scheduling:
     the device with the other workers';
   * every result is identical to labelling the request alone.
 
-Footprint of a request (an upper bound, reserved at admission and released at
-completion): labelling workspace + uint32 labels + the dusted copy + the
-counts and bounding boxes for up to one label per voxel.  Results belong to the caller
-after completion; the budget should leave room for the results callers keep.
+Footprint of a request: at admission, the labelling workspace + uint32 labels
++ the dusted copy.  Once the labelling has returned N, the workspace is given
+back and the per-label outputs (voxel counts and bounding boxes, 28 bytes for
+each of N + 1 labels) are reserved in its place; when they need more than the
+workspace gave back, the worker waits for the difference, unless no other
+request is in service (then it proceeds, so the service cannot stall).  The
+rest is released at completion.  Results belong to the caller after
+completion; the budget should leave room for the results callers keep.
 """
 from __future__ import annotations
 
@@ -65,11 +69,19 @@ class Request:
 
 
 def request_footprint(volume, connectivity: int) -> int:
-  """HBM bytes reserved for one request (see module docstring)."""
+  """HBM bytes reserved for one request at admission (see module docstring)."""
+  return _workspace(volume, connectivity) + _outputs(volume)
+
+
+def _workspace(volume, connectivity: int) -> int:
+  sx, sy, sz = _dims(volume)
+  return int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, 0, 0))
+
+
+def _outputs(volume) -> int:
   sx, sy, sz = _dims(volume)
   V = sx * sy * sz
-  ws = int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, 0, 0))
-  return ws + 4 * V + volume.element_size() * V + _STAT_BYTES_PER_LABEL * (V + 1)
+  return 4 * V + volume.element_size() * V  # labels + dusted copy
 
 
 def _dims(volume):
@@ -115,6 +127,9 @@ class LabelServer:
     self.dust_threshold = int(dust_threshold)
     self.cv = threading.Condition()
     self.waiting = collections.deque()
+    self.active = 0  # requests in service
+    self.stats_waiters = 0  # of those, requests waiting for their per-label outputs' memory
+    self.peak_reserved = 0  # the most HBM reserved at once
     self.closing = False
     self.streams = [torch.cuda.Stream() for _ in range(workers)]
     self.threads = [threading.Thread(target=self._worker, args=(s,), daemon=True) for s in self.streams]
@@ -150,6 +165,9 @@ class LabelServer:
     for th in self.threads:
       th.join()
 
+  def _note(self):
+    self.peak_reserved = max(self.peak_reserved, self.budget - self.free)
+
   def _small(self, r: Request) -> bool:
     sx, sy, sz = _dims(r.volume)
     return sx * sy * sz <= self.small_voxels
@@ -164,20 +182,45 @@ class LabelServer:
         self.cv.wait()
       head = self.waiting.popleft()
       self.free -= head.footprint
+      self._note()
       group = [head]
+      self.active += 1
       if self._small(head):
         keep = collections.deque()
         while self.waiting and len(group) < self.max_batch:
           r = self.waiting.popleft()
           if self._small(r) and r.footprint <= self.free and r.volume.dtype == head.volume.dtype:
             self.free -= r.footprint
+            self._note()
             group.append(r)
+            self.active += 1
           else:
             keep.append(r)
             break  # first come, first served: stop at the first request that does not join
         while keep:
           self.waiting.appendleft(keep.pop())
       return group
+
+  def _swap_for_stats(self, group: List[Request], ns: List[int]):
+    """The labelling workspace of each request, given back, for its
+    per-label outputs (see module docstring).  A group waits for memory only
+    while some request in service is not waiting too, so the service always
+    progresses."""
+    with self.cv:
+      self.stats_waiters += len(group)
+      self.cv.notify_all()
+      for r, n in zip(group, ns):
+        ws = _workspace(r.volume, r.connectivity)
+        stats = _STAT_BYTES_PER_LABEL * (n + 1)
+        self.free += ws
+        r.footprint -= ws
+        while stats > self.free and self.active > self.stats_waiters:
+          self.cv.wait()
+        self.free -= stats
+        self._note()
+        r.footprint += stats
+      self.stats_waiters -= len(group)
+      self.cv.notify_all()
 
   def _worker(self, stream):
     torch = _torch()
@@ -200,6 +243,7 @@ class LabelServer:
             lab = connected_components_batch([r.volume for r in group],
                                              connectivity=[r.connectivity for r in group],
                                              out_dtype=np.uint32, return_N=True)
+          self._swap_for_stats(group, [int(N) for _, N in lab])
           res = [_followons(r.volume, labels, int(N), self.dust_threshold) for r, (labels, N) in zip(group, lab)]
           stream.synchronize()
         for r, out in zip(group, res):
@@ -215,6 +259,7 @@ class LabelServer:
         for r in group:
           r.t_done = t1
           self.free += r.footprint
+          self.active -= 1
         self.cv.notify_all()
       for r in group:
         r.done.set()
