@@ -35,11 +35,12 @@ def _batchable(pl) -> bool:
 
 
 def _volume_table(plans, bufs):
+  from ._core import _tile_flags
   vt = (L.Volume * len(plans))()
   for k, (pl, buf) in enumerate(zip(plans, bufs)):
     vt[k].inp = buf.data_ptr()
     vt[k].sx, vt[k].sy, vt[k].sz = pl.sx, pl.sy, pl.sz
-    vt[k].connectivity = pl.connectivity
+    vt[k].connectivity = pl.connectivity | (_tile_flags(pl, buf) << 8)
   return vt
 
 

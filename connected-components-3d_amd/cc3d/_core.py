@@ -378,7 +378,8 @@ def _label_one(pl):
   torch = _torch()
   buf = pl.vol.device_buffer(pl.order)
   sx, sy, sz, connectivity = pl.sx, pl.sy, pl.sz, pl.connectivity
-  ws_bytes = int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, int(pl.binary_image), pl.flags))
+  flags = pl.flags | _tile_flags(pl, buf)
+  ws_bytes = int(L.lib().cc3d_amd_workspace_bytes(sx, sy, sz, connectivity, int(pl.binary_image), flags))
   if ws_bytes == 0:
     L.check(L.ERR_RUNTIME if connectivity in (4, 8) and sz != 1 else L.ERR_INVALID)
   ws = torch.empty(ws_bytes, dtype=torch.uint8, device=buf.device)
@@ -387,7 +388,7 @@ def _label_one(pl):
   dbuf = ctypes.create_string_buffer(pl.delta_b, max(len(pl.delta_b), 8))
   L.check(L.lib().cc3d_amd_label_begin(
     ctypes.byref(job), ctypes.c_void_p(buf.data_ptr()), pl.code, sx, sy, sz, connectivity,
-    ctypes.cast(dbuf, ctypes.c_void_p), pl.flags, int(pl.periodic_boundary), int(pl.binary_image),
+    ctypes.cast(dbuf, ctypes.c_void_p), flags, int(pl.periodic_boundary), int(pl.binary_image),
     ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream))
   try:
     st = L.Stats()
@@ -402,6 +403,16 @@ def _label_one(pl):
     N = _special_row(pl, buf, odt, out_dev)
 
   return _deliver(out_dev, pl.vol, sx, sy, sz, pl.dims, pl.order, odt, pl.out_file, N, pl.return_N)
+
+
+def _tile_flags(pl, buf) -> int:
+  """Tile family of the local kernel for this volume (cc3d/_tune.py)."""
+  from . import _tune
+  if pl.flags & L.FLAG_CONTINUOUS:
+    return 0
+  isz = pl.view_dtype.itemsize
+  return _tune.tiles(pl.connectivity, pl.binary_image, isz, pl.voxels, pl.sz > 1,
+                     lambda: _tune.sample_density(buf, isz, pl.voxels))
 
 
 _ROW_FAILURES = {

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libcc3d_amd.so")
 
 OK, ERR_RUNTIME, ERR_INVALID, ERR_HIP, ERR_NOMEM = 0, 1, 2, 3, 4
 FLAG_CONTINUOUS, FLAG_COUNT_PROVISIONAL, FLAG_NO_SINGLE_ROW_RULE = 1, 2, 4
+FLAG_TILES_ROWS, FLAG_TILES_VOXELS = 8, 16
 
 # dtype codes of the C ABI
 DT_U8, DT_U16, DT_U32, DT_U64, DT_F32, DT_F64 = 0, 1, 2, 3, 4, 5

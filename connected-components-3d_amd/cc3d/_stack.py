@@ -96,7 +96,11 @@ class _Streamer:
     torch = self.torch
     lib = L.lib()
     sx, sy = self.sx, self.sy
-    ws_bytes = int(lib.cc3d_amd_workspace_bytes(sx, sy, nz, self.connectivity, int(self.binary), self.flags))
+    from . import _tune
+    isz = dev_bytes.numel() // (sx * sy * nz)
+    flags = self.flags | _tune.tiles(self.connectivity, self.binary, isz, sx * sy * nz, nz > 1,
+                                     lambda: _tune.sample_density(dev_bytes, isz, sx * sy * nz))
+    ws_bytes = int(lib.cc3d_amd_workspace_bytes(sx, sy, nz, self.connectivity, int(self.binary), flags))
     if ws_bytes == 0:
       L.check(L.ERR_INVALID)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
@@ -105,7 +109,7 @@ class _Streamer:
     stream = _stream()
     L.check(lib.cc3d_amd_label_begin(
       ctypes.byref(job), _ptr(dev_bytes), self.code, sx, sy, nz, self.connectivity, ctypes.cast(dbuf, ctypes.c_void_p),
-      self.flags | L.FLAG_NO_SINGLE_ROW_RULE, 0, int(self.binary), _ptr(ws), ws_bytes, stream))
+      flags | L.FLAG_NO_SINGLE_ROW_RULE, 0, int(self.binary), _ptr(ws), ws_bytes, stream))
     try:
       st = L.Stats()
       L.check(lib.cc3d_amd_job_stats(job, ctypes.byref(st)))

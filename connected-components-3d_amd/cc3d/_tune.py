@@ -1,0 +1,212 @@
+"""Tile-family choice for the local labelling kernel, from a measured table.
+
+The local kernel comes in two tile families with identical results: voxel
+tiles (32x8x8, one union-find node per voxel; csrc/local.hpp) and row tiles
+(64x8x8, x-runs as nodes; csrc/rows.hpp).  Which is faster depends on the
+connectivity, the voxel type, the volume's size and how much of it is
+foreground.  `tile_table.json` (next to this file) holds, per key
+
+    connectivity / mode / voxel bytes / size class / density class
+
+the family measured faster and both timings.  `python -m cc3d._tune` measures
+every key on this GPU with synthetic volumes (Voronoi labellings, optionally
+thinned by a Bernoulli mask, and Bernoulli binary images) and rewrites the
+table.  At call time the key's size class comes from the shape; the density
+class is sampled from the device volume (4096 evenly spaced voxels, one
+small reduction) only when the table's choice for that key depends on it.
+Keys missing from the table keep the default family (rows for 6, voxels
+otherwise).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+import zlib
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib as L
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE_PATH = os.path.join(_HERE, "tile_table.json")
+
+SIZE_CLASSES = ((1 << 19, "s"), (1 << 23, "m"), (None, "l"))  # voxels: <= 512 Ki, <= 8 Mi, more
+DENSITY_CLASSES = ((0.3, "sparse"), (0.6, "mid"), (None, "dense"))  # foreground fraction
+_FLAG = {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS}
+
+_table = None
+# the tuner's own override: "rows" / "voxels" forces a family for every call
+FORCE: Optional[str] = None
+
+
+def size_class(voxels: int) -> str:
+  for lim, name in SIZE_CLASSES:
+    if lim is None or voxels <= lim:
+      return name
+  return SIZE_CLASSES[-1][1]
+
+
+def density_class(frac: float) -> str:
+  for lim, name in DENSITY_CLASSES:
+    if lim is None or frac < lim:
+      return name
+  return DENSITY_CLASSES[-1][1]
+
+
+def table() -> dict:
+  global _table
+  if _table is None:
+    try:
+      with open(TABLE_PATH) as f:
+        _table = json.load(f).get("entries", {})
+    except (OSError, ValueError):
+      _table = {}
+  return _table
+
+
+def key(connectivity: int, binary: bool, itemsize: int, voxels: int) -> str:
+  return f"{int(connectivity)}/{'binary' if binary else 'multi'}/{int(itemsize)}/{size_class(voxels)}"
+
+
+def sample_density(buf, itemsize: int, voxels: int) -> float:
+  """Fraction of nonzero voxels among up to 4096 evenly spaced ones of a
+  device buffer (raw bytes, itemsize bytes per voxel)."""
+  torch = _torch_mod()
+  n = min(4096, voxels)
+  step = max(1, voxels // n)
+  words = {1: torch.uint8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[itemsize]
+  flat = buf.view(torch.uint8).view(words) if buf.dtype != words else buf
+  pick = flat.as_strided((n,), (step,))
+  return float((pick != 0).sum().item()) / n
+
+
+def _torch_mod():
+  import torch
+  return torch
+
+
+def tiles(connectivity: int, binary: bool, itemsize: int, voxels: int, is3d: bool,
+          density: Callable[[], float]) -> int:
+  """CC3D_AMD_TILES_* flag bits for one volume (0: the default family)."""
+  if FORCE is not None:
+    return _FLAG[FORCE]
+  if not is3d:
+    return 0
+  row = table().get(key(connectivity, binary, itemsize, voxels))
+  if not row:
+    return 0
+  choices = {v["choice"] for v in row.values()}
+  if len(choices) == 1:
+    return _FLAG[choices.pop()]
+  d = row.get(density_class(density()))
+  return _FLAG[d["choice"]] if d else 0
+
+
+# ---------------------------------------------------------------------------
+# the measurement that writes the table
+# ---------------------------------------------------------------------------
+_SHAPES = {"s": (80, 80, 80), "m": (192, 192, 192), "l": (384, 384, 384)}
+_DENSITY_P = {"sparse": 0.22, "mid": 0.5, "dense": None}  # Bernoulli mask / foreground probability
+
+
+def _volume(synth, mode: str, itemsize: int, shape, density: str, seed: int) -> np.ndarray:
+  if mode == "binary":
+    p = {"sparse": 0.15, "mid": 0.45, "dense": 0.75}[density]
+    return synth.bernoulli(shape, p, seed=seed)
+  nseeds = max(1, int(np.prod(shape)) // 8000)
+  dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[itemsize]
+  vol = synth.voronoi(shape, nseeds, seed=seed, id_range=(1, None), dtype=dt) if itemsize == 8 else \
+    synth.voronoi(shape, nseeds, seed=seed).astype(dt)
+  keep = _DENSITY_P[density]
+  if keep is not None:
+    vol = vol * synth.bernoulli(shape, keep, seed=seed + 1).astype(dt)
+  return vol
+
+
+# The other family replaces the default one only when it is at least this much
+# faster (measured differences below it are within run-to-run noise).
+MARGIN = 0.02
+
+
+def decide(connectivity: int, ms_voxels: float, ms_rows: float) -> str:
+  default = "rows" if connectivity == 6 else "voxels"
+  other = "voxels" if default == "rows" else "rows"
+  ms = {"voxels": ms_voxels, "rows": ms_rows}
+  return other if ms[other] < (1.0 - MARGIN) * ms[default] else default
+
+
+def measure(reps: int = 5, log=print) -> dict:
+  """Times both tile families on every key; returns the table."""
+  import torch
+  import synth
+  from ._core import connected_components
+  global FORCE
+  entries = {}
+  started = time.time()
+  for conn in (26, 18, 6):
+    for mode, sizes in (("multi", (1, 2, 4, 8)), ("binary", (1,))):
+      for itemsize in sizes:
+        for cls, shape in _SHAPES.items():
+          k = f"{conn}/{mode}/{itemsize}/{cls}"
+          entries[k] = {}
+          for dens in ("sparse", "mid", "dense"):
+            vol = _volume(synth, mode, itemsize, shape, dens, seed=zlib.crc32(f"{k}/{dens}".encode()) % 100000)
+            d = torch.from_numpy(np.ravel(vol, order="F")).cuda().as_strided(
+              shape, (1, shape[0], shape[0] * shape[1]))
+            frac = float(np.count_nonzero(vol)) / vol.size
+            ms = {}
+            for fam in ("voxels", "rows"):
+              FORCE = fam
+              try:
+                for _ in range(2):
+                  connected_components(d, connectivity=conn, out_dtype=np.uint32, binary_image=mode == "binary")
+                torch.cuda.synchronize()
+                times = []
+                for _ in range(reps):
+                  a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                  a.record()
+                  connected_components(d, connectivity=conn, out_dtype=np.uint32, binary_image=mode == "binary")
+                  b.record()
+                  torch.cuda.synchronize()
+                  times.append(a.elapsed_time(b))
+                ms[fam] = float(np.median(times))
+              finally:
+                FORCE = None
+            choice = decide(conn, ms["voxels"], ms["rows"])
+            entries[k][density_class(frac)] = {"choice": choice, "ms_voxels": round(ms["voxels"], 4),
+                                               "ms_rows": round(ms["rows"], 4), "foreground": round(frac, 3)}
+            log(f"[tune {time.time() - started:6.1f}s] {k} {dens} fg={frac:.2f} voxels {ms['voxels']:.3f} ms "
+                f"rows {ms['rows']:.3f} ms -> {choice}")
+            del d
+  return entries
+
+
+def main(argv=None) -> int:
+  import argparse
+  import sys
+  ap = argparse.ArgumentParser(description="measure the tile families and write the tile table")
+  ap.add_argument("--out", default=TABLE_PATH)
+  ap.add_argument("--reps", type=int, default=5)
+  args = ap.parse_args(argv)
+  here = os.path.dirname(_HERE)
+  if here not in sys.path:
+    sys.path.insert(0, here)
+  entries = measure(args.reps, log=lambda m: print(m, flush=True))
+  import torch
+  doc = {"device": torch.cuda.get_device_name(0), "reps": args.reps,
+         "note": "median ms of one connected_components call (device input, uint32 out) per tile family",
+         "entries": entries}
+  with open(args.out, "w") as f:
+    json.dump(doc, f, indent=1, sort_keys=True)
+  print("wrote", args.out)
+  return 0
+
+
+if __name__ == "__main__":
+  # run as the package's module, so FORCE is the flag the dispatcher reads
+  import importlib
+  import sys
+  sys.path.insert(0, os.path.dirname(_HERE))
+  raise SystemExit(importlib.import_module("cc3d._tune").main())

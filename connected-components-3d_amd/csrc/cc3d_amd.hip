@@ -1138,8 +1138,9 @@ struct Geometry {
   size_t oGP, oRL, oTR, oTQ, oTC, oJ8, oJ16, oREQ, oBITS, oPRE, oBSUM, oCTR, oMM, oQB, oQP, oQS, total;
 };
 
+// tiles: CC3D_AMD_TILES_ROWS / _VOXELS bits (else the default family)
 static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary,
-                int continuous) {
+                int continuous, int tiles = 0) {
   if (sx <= 0 || sy <= 0 || sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
   if (connectivity != 26 && connectivity != 18 && connectivity != 6 && connectivity != 8 && connectivity != 4)
     return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
@@ -1168,7 +1169,8 @@ static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connect
   int TX, TY, TZ;
   // Row tiles for 6-connected 3D volumes (multilabel or binary): measured
   // faster there than the voxel tiles; the voxel tiles stay faster at 26/18.
-  g.rows = g.is3d && g.mode != M_CONT && connectivity == 6;
+  g.rows = g.is3d && g.mode != M_CONT &&
+           ((tiles & CC3D_AMD_TILES_ROWS) ? true : (tiles & CC3D_AMD_TILES_VOXELS) ? false : connectivity == 6);
   if (g.rows) {
     TX = GeoR::TX; TY = GeoR::TY; TZ = GeoR::TZ; g.TV = GeoR::TV; g.HALO = GeoR::HALO;
   } else if (g.is3d) {
@@ -1403,7 +1405,8 @@ const char* cc3d_amd_version(void) { return "cc3d_amd 0.1.0 (gfx950)"; }
 size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary_image,
                                 int flags) {
   Geometry g;
-  if (plan(g, sx, sy, sz, connectivity, binary_image, (flags & CC3D_AMD_CONTINUOUS) != 0) != CC3D_AMD_OK) return 0;
+  if (plan(g, sx, sy, sz, connectivity, binary_image, (flags & CC3D_AMD_CONTINUOUS) != 0, flags) != CC3D_AMD_OK)
+    return 0;
   return g.total;
 }
 
@@ -1420,7 +1423,7 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
   if (in_dtype < DT_U8 || in_dtype > DT_F64) return fail(CC3D_AMD_ERR_INVALID, "unsupported input dtype");
   if (!in) return fail(CC3D_AMD_ERR_INVALID, "input pointer is NULL");
   const bool continuous = (flags & CC3D_AMD_CONTINUOUS) && !binary_image;
-  int rc = plan(j->g, sx, sy, sz, connectivity, binary_image, continuous);
+  int rc = plan(j->g, sx, sy, sz, connectivity, binary_image, continuous, flags);
   if (rc != CC3D_AMD_OK) return rc;
   if (periodic_boundary && j->g.mode == M_CONT)
     return fail(CC3D_AMD_ERR_RUNTIME, "periodic_boundary is not currently supported for continuous data.");
@@ -1822,11 +1825,14 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   b.perm.resize(n);
   b.groups.clear();
   std::vector<int> nbs(n);
+  std::vector<char> rowsv(n);
   for (int i = 0; i < n; i++) {  // checked in the caller's order
     const cc3d_amd_volume& v = vols[i];
     if (!v.in) return fail(CC3D_AMD_ERR_INVALID, "NULL input in batch");
     if (v.sx <= 0 || v.sy <= 0 || v.sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
-    int nb = (int)v.connectivity;
+    int nb = (int)(v.connectivity & 0xFF);
+    if ((v.connectivity & ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS) << 8 | 0xFF)) != 0)
+      return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     if (nb == 8 || nb == 4) {
       if (v.sz != 1) return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
       if (binary) return fail(CC3D_AMD_ERR_INVALID, "binary 2D volumes are not supported in a batch");
@@ -1837,14 +1843,19 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     }
     nbs[i] = nb;
+    const int pref = (int)(v.connectivity >> 8);
+    rowsv[i] = (pref & CC3D_AMD_TILES_ROWS) ? true : (pref & CC3D_AMD_TILES_VOXELS) ? false : nb == 6;
     b.perm[i] = i;
   }
-  std::stable_sort(b.perm.begin(), b.perm.end(), [&](int x, int y) { return nbs[x] > nbs[y]; });
+  // voxel-tile volumes first, then row-tile ones; within each, 26, 18, 6
+  std::stable_sort(b.perm.begin(), b.perm.end(), [&](int x, int y) {
+    return rowsv[x] != rowsv[y] ? !rowsv[x] : nbs[x] > nbs[y];
+  });
   uint64_t key = 0;
   b.tgroups.clear();
   for (int k = 0; k < n; k++) {
     const int i = b.perm[k], nb = nbs[i];
-    const bool rows = nb == 6;
+    const bool rows = rowsv[i] != 0;
     if (b.tgroups.empty() || b.tgroups.back().rows != rows) {
       cc3d_amd_batch::TileGroup tg;
       tg.rows = rows;
@@ -1871,7 +1882,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
     t.nkeys = (uint32_t)std::min<uint64_t>(V, 0xFFFFFFFFull);
     t.nb = nb;
     t.out_dtype = DT_U32;
-    if (b.groups.empty() || b.groups.back().nb != nb)
+    if (b.groups.empty() || b.groups.back().nb != nb || b.groups.back().tg != (int)b.tgroups.size() - 1)
       b.groups.push_back({nb, (int)b.tgroups.size() - 1, tg.tiles, 0});
     b.groups.back().tiles += t.ntiles;
     tg.tiles += t.ntiles;

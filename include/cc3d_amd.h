@@ -107,6 +107,12 @@ typedef struct {
 #define CC3D_AMD_CONTINUOUS 1          /* |a-b| <= delta linking (the delta != 0 path) */
 #define CC3D_AMD_COUNT_PROVISIONAL 2   /* binary 2D-8: count provisional block labels */
 #define CC3D_AMD_NO_SINGLE_ROW_RULE 4  /* do not mimic the binding's single-row shortcut */
+/* Tile family of the local labelling kernel for 3D volumes (no reference
+ * counterpart: a performance choice, results are identical): row tiles
+ * (64x8x8, x-runs as union-find nodes) or voxel tiles (32x8x8, one node per
+ * voxel).  Neither: row tiles for 6-connectivity, voxel tiles otherwise. */
+#define CC3D_AMD_TILES_ROWS 8
+#define CC3D_AMD_TILES_VOXELS 16
 
 /* Bytes of scratch a job of this geometry needs (0 on bad arguments). */
 size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity,
@@ -149,7 +155,7 @@ uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t 
 typedef struct {
   const void* in;          /* device pointer, dense, x fastest */
   int64_t sx, sy, sz;
-  int64_t connectivity;
+  int64_t connectivity;    /* | (CC3D_AMD_TILES_ROWS or _VOXELS) << 8 to choose the tile family */
 } cc3d_amd_volume;
 
 typedef struct cc3d_amd_batch cc3d_amd_batch;

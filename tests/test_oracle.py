@@ -477,3 +477,38 @@ def test_oracle_each_known_answer():
         return oracle.each(labels, binary=binary, in_place=in_place)
     return It()
   ka.check_each(each, side=24)
+
+
+def test_tile_table_lookup():
+  """The tile-family table: every key present with both timings, choices
+  following its rule, and the dispatcher's lookup (density sampled only
+  when the choice depends on it)."""
+  import json
+  import torch
+  from cc3d import _lib as L
+  from cc3d import _tune
+  with open(_tune.TABLE_PATH) as f:
+    doc = json.load(f)
+  ent = doc["entries"]
+  for conn in (26, 18, 6):
+    for mode, sizes in (("multi", (1, 2, 4, 8)), ("binary", (1,))):
+      for isz in sizes:
+        for cls in ("s", "m", "l"):
+          row = ent[f"{conn}/{mode}/{isz}/{cls}"]
+          for e in row.values():
+            assert e["choice"] == _tune.decide(conn, e["ms_voxels"], e["ms_rows"])
+  calls = []
+
+  def density():
+    calls.append(1)
+    return 0.1
+  flag = _tune.tiles(6, False, 4, 400 ** 3, True, density)
+  row = ent[_tune.key(6, False, 4, 400 ** 3)]
+  exp = row[_tune.density_class(0.1)]["choice"] if len({v["choice"] for v in row.values()}) > 1 else \
+    next(iter(row.values()))["choice"]
+  assert flag == {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS}[exp]
+  assert bool(calls) == (len({v["choice"] for v in row.values()}) > 1)
+  assert _tune.tiles(26, False, 4, 100, False, density) == 0  # a single plane: no choice
+  g = torch.Generator().manual_seed(0)
+  buf = (torch.rand(100000, generator=g) < 0.3).to(torch.int32) * 7
+  assert abs(_tune.sample_density(buf.view(torch.uint8), 4, 100000) - 0.3) < 0.04
