@@ -394,7 +394,7 @@ def main() -> int:
   ap.add_argument("--no-configs", action="store_true",
                   help="skip timing the other workloads at full size against the reference")
   ap.add_argument("--calib", action="store_true",
-                  help="also run one 256 MiB device copy (counter calibration for profiles)")
+                  help="also run one 2 GiB device copy (counter calibration for profiles)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -419,11 +419,15 @@ def main() -> int:
   torch.cuda.synchronize()
 
   if args.calib:
-    a = torch.empty(64 << 20, dtype=torch.int32, device=dev)
+    # one 2 GiB device copy of a written buffer, after a 1 GiB write that
+    # evicts it from the caches: HBM bytes the counters should report
+    a = torch.ones(512 << 20, dtype=torch.int32, device=dev)
+    e = torch.zeros(256 << 20, dtype=torch.int32, device=dev)
     b = torch.empty_like(a)
+    torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    del a, b
+    del a, b, e
 
   def step():
     return cc3d.connected_components(dvol, connectivity=26, out_dtype=np.uint32, return_N=True)
