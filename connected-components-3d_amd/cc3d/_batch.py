@@ -3,10 +3,11 @@
 No counterpart in the reference, which labels one volume per call.  Result i
 of `connected_components_batch(volumes, ...)` is exactly
 `connected_components(volumes[i], ...)`: the same labels, N, output dtype and
-exceptions.  Volumes the batched kernels cover (multilabel or binary, 3D
-connectivities or a single plane, no periodic boundary, no delta) share one
-launch of every kernel (cc3d_amd_batch_* in include/cc3d_amd.h); the rest go
-through the single-volume path.
+exceptions.  Volumes the batched kernels cover share one launch of every
+kernel (cc3d_amd_batch_* in include/cc3d_amd.h), one batch per input dtype and
+mode (multilabel, binary, continuous): 3D connectivities, or a single plane,
+with or without a periodic boundary.  The rest -- binary 2D-8 and 2D-4,
+continuous 2D-8 and out_file outputs -- go through the single-volume path.
 
 `LabelBatch` keeps a planned batch of device-resident volumes with fixed
 output buffers, for repeated labelling; it can capture the whole launch
@@ -24,14 +25,27 @@ from ._core import (_OUT_CODE, _decide, _deliver, _label_one, _prepare, _require
                     _torch, _torch_out_dtype)
 
 
+def _mode(pl) -> int:
+  if pl.binary_image:
+    return L.BATCH_BINARY
+  return L.BATCH_CONTINUOUS if pl.flags & L.FLAG_CONTINUOUS else L.BATCH_MULTILABEL
+
+
 def _batchable(pl) -> bool:
-  if pl.flags or pl.periodic_boundary or pl.out_file is not None:
+  if pl.out_file is not None or (pl.flags & ~L.FLAG_CONTINUOUS):
     return False
   if pl.connectivity in (26, 18, 6):
-    return not (pl.binary_image and pl.sz == 1)
+    return True
   if pl.connectivity in (8, 4):
-    return pl.sz == 1 and not pl.binary_image
+    mode = _mode(pl)
+    return pl.sz == 1 and mode != L.BATCH_BINARY and not (mode == L.BATCH_CONTINUOUS and pl.connectivity == 8)
   return False
+
+
+def _group_key(pl):
+  """Plans of one batch share input dtype, mode and (continuous) delta."""
+  mode = _mode(pl)
+  return pl.code, mode, (pl.delta_b if mode == L.BATCH_CONTINUOUS else b"")
 
 
 def _volume_table(plans, bufs):
@@ -40,26 +54,46 @@ def _volume_table(plans, bufs):
   for k, (pl, buf) in enumerate(zip(plans, bufs)):
     vt[k].inp = buf.data_ptr()
     vt[k].sx, vt[k].sy, vt[k].sz = pl.sx, pl.sy, pl.sz
-    vt[k].connectivity = pl.connectivity | (_tile_flags(pl, buf) << 8)
+    vt[k].connectivity = (pl.connectivity | (_tile_flags(pl, buf) << 8)
+                          | (L.VOLUME_PERIODIC if pl.periodic_boundary else 0))
   return vt
 
 
+def _create(plans, bufs, ws, ws_bytes, stream):
+  """cc3d_amd_batch_create (+ delta for a continuous batch) -> handle."""
+  lib = L.lib()
+  code, mode, delta_b = _group_key(plans[0])
+  vt = _volume_table(plans, bufs)
+  b = ctypes.c_void_p(0)
+  L.check(lib.cc3d_amd_batch_create(ctypes.byref(b), vt, len(plans), code, mode, ctypes.c_void_p(ws.data_ptr()),
+                                    ws_bytes, stream))
+  if mode == L.BATCH_CONTINUOUS:
+    dbuf = ctypes.create_string_buffer(delta_b, max(len(delta_b), 8))
+    rc = lib.cc3d_amd_batch_set_delta(b, ctypes.cast(dbuf, ctypes.c_void_p))
+    if rc != L.OK:
+      lib.cc3d_amd_batch_free(b)
+      L.check(rc)
+  return b
+
+
+def _workspace_bytes(plans, bufs) -> int:
+  ws_bytes = int(L.lib().cc3d_amd_batch_workspace_bytes(_volume_table(plans, bufs), len(plans),
+                                                         _group_key(plans[0])[1]))
+  if ws_bytes == 0:
+    L.check(L.ERR_INVALID)
+  return ws_bytes
+
+
 def _label_group(items, results, errors):
-  """Labels plans sharing input dtype and binary mode in one batch."""
+  """Labels plans sharing input dtype, mode and delta in one batch."""
   torch = _torch()
   plans = [pl for _, pl in items]
   bufs = [pl.vol.device_buffer(pl.order) for pl in plans]
   n = len(plans)
-  binary = int(plans[0].binary_image)
-  vt = _volume_table(plans, bufs)
   lib = L.lib()
-  ws_bytes = int(lib.cc3d_amd_batch_workspace_bytes(vt, n, binary))
-  if ws_bytes == 0:
-    L.check(L.ERR_INVALID)
+  ws_bytes = _workspace_bytes(plans, bufs)
   ws = torch.empty(ws_bytes, dtype=torch.uint8, device=bufs[0].device)
-  b = ctypes.c_void_p(0)
-  L.check(lib.cc3d_amd_batch_create(ctypes.byref(b), vt, n, plans[0].code, binary, ctypes.c_void_p(ws.data_ptr()),
-                                    ws_bytes, _stream()))
+  b = _create(plans, bufs, ws, ws_bytes, _stream())
   try:
     L.check(lib.cc3d_amd_batch_begin(b))
     st = (L.Stats * n)()
@@ -131,7 +165,7 @@ def connected_components_batch(
   groups = {}
   for i, pl in plans:
     if _batchable(pl):
-      groups.setdefault((pl.code, pl.binary_image), []).append((i, pl))
+      groups.setdefault(_group_key(pl), []).append((i, pl))
     else:
       try:
         results[i] = _label_one(pl)
@@ -148,44 +182,41 @@ class LabelBatch:
   """A planned batch of device-resident volumes with fixed output buffers.
 
   volumes: device torch tensors (any layout connected_components accepts).
-  All must be batchable and share dtype and binary mode; out_dtype is fixed
-  (uint16/uint32/uint64) so the whole sequence can run as one HIP graph.
+  All must be batchable and share dtype and mode (binary_image, delta);
+  out_dtype is fixed (uint16/uint32/uint64) so the whole sequence can run as
+  one HIP graph.
   `run()` labels through the two-stage launches; `capture()` records begin +
   finish in a graph and `replay()` relaunches it.  Both return
   [(labels, N)], raising like connected_components when a volume's estimate
   does not fit out_dtype or its provisional labels overflow.
   """
 
-  def __init__(self, volumes, connectivity=26, out_dtype=np.uint32, binary_image=False, stream=None):
+  def __init__(self, volumes, connectivity=26, out_dtype=np.uint32, binary_image=False, stream=None, delta=0,
+               periodic_boundary=False):
     torch = _torch()
     _require_gpu()
     n = len(volumes)
     conns = list(connectivity) if isinstance(connectivity, (list, tuple)) else [connectivity] * n
     self.plans = []
     for v, c in zip(volumes, conns):
-      kind, pl = _prepare(v, -1, c, 0, out_dtype, None, False, binary_image, True)
+      kind, pl = _prepare(v, -1, c, delta, out_dtype, None, periodic_boundary, binary_image, True)
       if kind != "plan" or not _batchable(pl):
         raise ValueError("LabelBatch volumes must be non-empty and batchable")
       self.plans.append(pl)
-    if len({(pl.code, pl.binary_image) for pl in self.plans}) != 1:
-      raise ValueError("LabelBatch volumes must share dtype and binary mode")
+    if len({_group_key(pl) for pl in self.plans}) != 1:
+      raise ValueError("LabelBatch volumes must share dtype and mode")
     self.odt = np.dtype(out_dtype)
     self.stream = stream if stream is not None else torch.cuda.Stream()
     self.bufs = [pl.vol.device_buffer(pl.order) for pl in self.plans]
     dev = self.bufs[0].device
     self.n = n
-    vt = _volume_table(self.plans, self.bufs)
     lib = L.lib()
-    ws_bytes = int(lib.cc3d_amd_batch_workspace_bytes(vt, n, int(binary_image)))
-    if ws_bytes == 0:
-      L.check(L.ERR_INVALID)
+    ws_bytes = _workspace_bytes(self.plans, self.bufs)
     self.ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     self.outs = [torch.empty(pl.voxels, dtype=_torch_out_dtype(self.odt), device=dev) for pl in self.plans]
     self.b = ctypes.c_void_p(0)
     torch.cuda.current_stream().synchronize()  # inputs and buffers are ready before the batch stream uses them
-    s = ctypes.c_void_p(self.stream.cuda_stream)
-    L.check(lib.cc3d_amd_batch_create(ctypes.byref(self.b), vt, n, self.plans[0].code, int(binary_image),
-                                      ctypes.c_void_p(self.ws.data_ptr()), ws_bytes, s))
+    self.b = _create(self.plans, self.bufs, self.ws, ws_bytes, ctypes.c_void_p(self.stream.cuda_stream))
     ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in self.outs])
     codes = (ctypes.c_int * n)(*([_OUT_CODE[self.odt]] * n))
     L.check(lib.cc3d_amd_batch_set_outputs(self.b, ptrs, codes))

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libcc3d_amd.so")
 OK, ERR_RUNTIME, ERR_INVALID, ERR_HIP, ERR_NOMEM = 0, 1, 2, 3, 4
 FLAG_CONTINUOUS, FLAG_COUNT_PROVISIONAL, FLAG_NO_SINGLE_ROW_RULE = 1, 2, 4
 FLAG_TILES_ROWS, FLAG_TILES_VOXELS = 8, 16
+VOLUME_PERIODIC = 1 << 16
+BATCH_MULTILABEL, BATCH_BINARY, BATCH_CONTINUOUS = 0, 1, 2
 
 # dtype codes of the C ABI
 DT_U8, DT_U16, DT_U32, DT_U64, DT_F32, DT_F64 = 0, 1, 2, 3, 4, 5
@@ -81,6 +83,8 @@ def lib():
   L.cc3d_amd_batch_workspace_bytes.argtypes = [ctypes.POINTER(Volume), ci, ci]
   L.cc3d_amd_batch_create.restype = ci
   L.cc3d_amd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Volume), ci, ci, ci, vp, ctypes.c_size_t, vp]
+  L.cc3d_amd_batch_set_delta.restype = ci
+  L.cc3d_amd_batch_set_delta.argtypes = [vp, vp]
   L.cc3d_amd_batch_set_outputs.restype = ci
   L.cc3d_amd_batch_set_outputs.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(ci)]
   for name in ("cc3d_amd_batch_begin", "cc3d_amd_batch_finish", "cc3d_amd_batch_graph_capture",

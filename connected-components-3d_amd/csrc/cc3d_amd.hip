@@ -165,6 +165,10 @@ struct GlobalParams {
   const int64_t* vtile0;
   const uint32_t* vix;
   int nvol;
+  // single volume (nvol == 0) placed at a tile and key offset: one volume of
+  // a batch addressed on its own (its periodic unions)
+  int64_t tile0;
+  uint32_t key0;
 };
 
 // Geometry of the volume owning batch tile `slot`.
@@ -187,8 +191,8 @@ __device__ inline VolGeo vol_geo(const GlobalParams& p, int64_t slot) {
   VolGeo g;
   g.sx = p.sx; g.sy = p.sy; g.sz = p.sz; g.osx = p.osx;
   g.ntx = p.ntx; g.nty = p.nty; g.ntz = p.ntz;
-  g.tile0 = 0;
-  g.key0 = 0;
+  g.tile0 = p.tile0;
+  g.key0 = p.key0;
   return g;
 }
 
@@ -1271,9 +1275,16 @@ struct cc3d_amd_batch {
     GlobalParams gp{};
     LocalParams lp{};
   };
+  // volumes with a periodic boundary: table entry, wrap neighbourhood (6, 4
+  // or 8, as the single-volume path picks it) and tile group
+  struct Periodic {
+    int k, nbp, tg;
+  };
   std::vector<int> perm;
   std::vector<Group> groups;
   std::vector<TileGroup> tgroups;
+  std::vector<Periodic> periodic;
+  bool delta_set = false;
   bool outputs_set = false, capturing = false;
   GlobalParams gp{};  // whole batch: the key space, bitmask, counters
   hipGraph_t graph = nullptr;
@@ -1393,6 +1404,58 @@ bool delta_is_zero(int dt, uint64_t bits) {
   if (dt == DT_F32) return from_bits<float>(bits) == 0.0f;
   if (dt == DT_F64) return from_bits<double>(bits) == 0.0;
   return bits == 0;
+}
+
+// Wrap-around unions of one periodic volume of a batch: the single-volume
+// kernel on that volume's tiles and keys (table entry k, tile group tg).
+template <class G>
+hipError_t batch_periodic(const cc3d_amd_batch* b, const cc3d_amd_batch::Periodic& pv, hipStream_t s) {
+  const BatchVol& v = b->h_tab[pv.k];
+  const auto& tg = b->tgroups[pv.tg];
+  GlobalParams q = tg.gp;
+  q.nvol = 0;
+  q.vols = nullptr;
+  q.vix = nullptr;
+  q.sx = v.sx;
+  q.sy = v.sy;
+  q.sz = v.sz;
+  q.osx = v.osx;
+  q.ntx = v.ntx;
+  q.nty = v.nty;
+  q.ntz = v.ntz;
+  q.tile0 = v.tile0;
+  q.key0 = v.key0;
+  q.ctr = b->gp.ctr + 1 + pv.k;
+  const int64_t npairs = pv.nbp == 6 ? (v.sy * v.sz + v.sx * v.sz + v.sx * v.sy)
+                         : pv.nbp == 4 ? (v.sx + v.sy) : (3 * v.sx + 2 + 3 * v.sy);
+  const dim3 grid(grid_for(npairs, WG, 65536));
+  // a multilabel volume whose foreground is one row skips its wrap unions, as
+  // cc3d_amd_label_begin does by default (the volume's own counters decide)
+  const int sr = b->mode == M_MULTI;
+#define CC3D_PB(T, M)                                                                                                \
+  do {                                                                                                               \
+    const T* in = static_cast<const T*>(v.in);                                                                       \
+    if (pv.nbp == 6) hipLaunchKernelGGL((k_periodic<T, M, 6, G>), grid, dim3(WG), 0, s, q, in, sr);                  \
+    else if (pv.nbp == 4) hipLaunchKernelGGL((k_periodic<T, M, 4, G>), grid, dim3(WG), 0, s, q, in, sr);             \
+    else hipLaunchKernelGGL((k_periodic<T, M, 8, G>), grid, dim3(WG), 0, s, q, in, sr);                              \
+  } while (0)
+#define CC3D_PBT(T)                           \
+  do {                                        \
+    if (b->mode == M_BINARY) CC3D_PB(T, M_BINARY); \
+    else CC3D_PB(T, M_MULTI);                 \
+  } while (0)
+  switch (b->dt) {
+    case DT_U8: CC3D_PBT(uint8_t); break;
+    case DT_U16: CC3D_PBT(uint16_t); break;
+    case DT_U32: CC3D_PBT(uint32_t); break;
+    case DT_U64: CC3D_PBT(uint64_t); break;
+    case DT_F32: CC3D_PBT(float); break;
+    case DT_F64: CC3D_PBT(double); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef CC3D_PBT
+#undef CC3D_PB
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -1817,34 +1880,55 @@ int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* c
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
-static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vector<BatchVol>& tab,
+// Volumes of a batch run the single-volume path's kernels on their own tiles.
+// A single plane is labelled on the 3D tiles: its 26/18-neighbourhood within
+// the plane is the 8-neighbourhood and its 6-neighbourhood the
+// 4-neighbourhood, so 8 and 4 map to 26 and 6.  Not covered: binary 2D-8 and
+// 2D-4 (their block and first-row numberings) and continuous 2D-8 (its global
+// min/max rule).
+static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vector<BatchVol>& tab,
                         std::vector<int64_t>& t0, cc3d_amd_batch& b) {
   if (!vols || n <= 0) return fail(CC3D_AMD_ERR_INVALID, "a batch needs at least one volume");
+  if (mode != CC3D_AMD_BATCH_MULTILABEL && mode != CC3D_AMD_BATCH_BINARY && mode != CC3D_AMD_BATCH_CONTINUOUS)
+    return fail(CC3D_AMD_ERR_INVALID, "batch mode must be multilabel, binary or continuous");
   tab.resize(n);
   t0.resize(n + 2);
   b.perm.resize(n);
   b.groups.clear();
-  std::vector<int> nbs(n);
+  std::vector<int> nbs(n), nbps(n);
   std::vector<char> rowsv(n);
   for (int i = 0; i < n; i++) {  // checked in the caller's order
     const cc3d_amd_volume& v = vols[i];
     if (!v.in) return fail(CC3D_AMD_ERR_INVALID, "NULL input in batch");
     if (v.sx <= 0 || v.sy <= 0 || v.sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
-    int nb = (int)(v.connectivity & 0xFF);
-    if ((v.connectivity & ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS) << 8 | 0xFF)) != 0)
+    const int conn = (int)(v.connectivity & 0xFF);
+    int nb = conn;
+    if ((v.connectivity &
+         ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS) << 8 | CC3D_AMD_VOLUME_PERIODIC | 0xFF)) != 0)
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     if (nb == 8 || nb == 4) {
       if (v.sz != 1) return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
-      if (binary) return fail(CC3D_AMD_ERR_INVALID, "binary 2D volumes are not supported in a batch");
+      if (mode == CC3D_AMD_BATCH_BINARY)
+        return fail(CC3D_AMD_ERR_INVALID, "binary 2D-8 and 2D-4 volumes are not supported in a batch");
+      if (mode == CC3D_AMD_BATCH_CONTINUOUS && nb == 8)
+        return fail(CC3D_AMD_ERR_INVALID, "continuous 2D-8 volumes are not supported in a batch");
       nb = nb == 8 ? 26 : 6;  // a single plane: the same links
-    } else if (nb == 26 || nb == 18 || nb == 6) {
-      if (binary && v.sz == 1) return fail(CC3D_AMD_ERR_INVALID, "binary 2D volumes are not supported in a batch");
-    } else {
+    } else if (nb != 26 && nb != 18 && nb != 6) {
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     }
+    nbps[i] = 0;
+    if (v.connectivity & CC3D_AMD_VOLUME_PERIODIC) {
+      if (mode == CC3D_AMD_BATCH_CONTINUOUS)
+        return fail(CC3D_AMD_ERR_INVALID, "periodic_boundary is not implemented for continuous data");
+      // the single-volume path's wrap neighbourhood: 6 in 3D, else the plane's
+      nbps[i] = v.sz > 1 ? 6 : (conn == 26 || conn == 18) ? 8 : conn == 6 ? 4 : conn;
+    }
     nbs[i] = nb;
-    const int pref = (int)(v.connectivity >> 8);
-    rowsv[i] = (pref & CC3D_AMD_TILES_ROWS) ? true : (pref & CC3D_AMD_TILES_VOXELS) ? false : nb == 6;
+    const int pref = (int)((v.connectivity >> 8) & 0xFF);
+    rowsv[i] = mode == CC3D_AMD_BATCH_CONTINUOUS ? false
+               : (pref & CC3D_AMD_TILES_ROWS)    ? true
+               : (pref & CC3D_AMD_TILES_VOXELS)  ? false
+                                                 : nb == 6;
     b.perm[i] = i;
   }
   // voxel-tile volumes first, then row-tile ones; within each, 26, 18, 6
@@ -1853,6 +1937,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   });
   uint64_t key = 0;
   b.tgroups.clear();
+  b.periodic.clear();
   for (int k = 0; k < n; k++) {
     const int i = b.perm[k], nb = nbs[i];
     const bool rows = rowsv[i] != 0;
@@ -1885,6 +1970,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
     if (b.groups.empty() || b.groups.back().nb != nb || b.groups.back().tg != (int)b.tgroups.size() - 1)
       b.groups.push_back({nb, (int)b.tgroups.size() - 1, tg.tiles, 0});
     b.groups.back().tiles += t.ntiles;
+    if (nbps[i]) b.periodic.push_back({k, nbps[i], (int)b.tgroups.size() - 1});
     tg.tiles += t.ntiles;
     tg.v1 = k + 1;
     key += (V + 31) & ~(uint64_t)31;
@@ -1932,29 +2018,29 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int binary, std::vec
   return CC3D_AMD_OK;
 }
 
-size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int binary_image) {
+size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int mode) {
   std::vector<BatchVol> tab;
   std::vector<int64_t> t0;
   cc3d_amd_batch b;
-  if (batch_layout(vols, n, binary_image, tab, t0, b) != CC3D_AMD_OK) return 0;
+  if (batch_layout(vols, n, mode, tab, t0, b) != CC3D_AMD_OK) return 0;
   return b.total;
 }
 
-int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vols, int n, int in_dtype,
-                          int binary_image, void* workspace, size_t workspace_bytes, void* stream) {
+int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vols, int n, int in_dtype, int mode,
+                          void* workspace, size_t workspace_bytes, void* stream) {
   if (!out_batch) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
   *out_batch = nullptr;
   if (in_dtype < DT_U8 || in_dtype > DT_F64) return fail(CC3D_AMD_ERR_INVALID, "unsupported input dtype");
   std::vector<BatchVol> tab;
   std::vector<int64_t> t0;
   cc3d_amd_batch* b = new cc3d_amd_batch();
-  int rc = batch_layout(vols, n, binary_image, tab, t0, *b);
+  int rc = batch_layout(vols, n, mode, tab, t0, *b);
   if (rc != CC3D_AMD_OK) {
     delete b;
     return rc;
   }
   b->dt = in_dtype;
-  b->mode = binary_image ? M_BINARY : M_MULTI;
+  b->mode = mode == CC3D_AMD_BATCH_BINARY ? M_BINARY : mode == CC3D_AMD_BATCH_CONTINUOUS ? M_CONT : M_MULTI;
   b->stream = static_cast<hipStream_t>(stream);
   if (workspace) {
     if (workspace_bytes < b->total) {
@@ -2073,8 +2159,21 @@ int cc3d_amd_batch_set_outputs(cc3d_amd_batch* b, void* const* outs, const int* 
   return CC3D_AMD_OK;
 }
 
+int cc3d_amd_batch_set_delta(cc3d_amd_batch* b, const void* delta) {
+  if (!b || !delta) return fail(CC3D_AMD_ERR_INVALID, "NULL argument");
+  if (b->mode != M_CONT) return fail(CC3D_AMD_ERR_INVALID, "delta applies to continuous batches only");
+  uint64_t bits = 0;
+  const int rc = delta_bits_of(b->dt, delta, bits);
+  if (rc != CC3D_AMD_OK) return rc;
+  for (auto& tg : b->tgroups) tg.lp.delta_bits = bits;
+  b->delta_set = true;
+  return CC3D_AMD_OK;
+}
+
 int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   if (!b) return fail(CC3D_AMD_ERR_INVALID, "NULL batch");
+  if (b->mode == M_CONT && !b->delta_set)
+    return fail(CC3D_AMD_ERR_INVALID, "a continuous batch needs cc3d_amd_batch_set_delta before it runs");
   hipStream_t s = b->stream;
   const GlobalParams& p = b->gp;
   {
@@ -2094,6 +2193,8 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
         e = launch_rows_batch_multi(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else if (b->mode == M_BINARY)
         e = launch_local_batch_binary(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+      else if (b->mode == M_CONT)
+        e = launch_local_batch_cont(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else
         e = launch_local_batch_multi(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       if (e != hipSuccess) break;
@@ -2113,6 +2214,14 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
       const unsigned blocks = grid_for(tg.tiles, WG / TILE_LANES, 65536);
       if (tg.rows) hipLaunchKernelGGL((k_requests<GeoR, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  if (!b->periodic.empty()) {
+    ProfScope ps_("k_periodic", s);
+    for (const auto& pv : b->periodic) {
+      e = b->tgroups[pv.tg].rows ? batch_periodic<GeoR>(b, pv, s) : batch_periodic<Geo3>(b, pv, s);
+      if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_periodic: ") + hipGetErrorString(e));
     }
   }
   {

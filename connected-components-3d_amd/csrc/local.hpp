@@ -1242,6 +1242,8 @@ hipError_t launch_local_batch_multi(int dt, int nb, const LocalParams& p, const 
                                     int64_t gtile0, int64_t gtiles, hipStream_t s);
 hipError_t launch_local_batch_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
                                      int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_cont(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
+                                   int64_t gtile0, int64_t gtiles, hipStream_t s);
 
 template <int MODE>
 hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,

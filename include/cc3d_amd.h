@@ -145,28 +145,41 @@ uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t 
 
 /* ---- batches: many volumes labelled by one set of launches ---------------
  * No reference counterpart: the reference labels one volume per call.  Each
- * volume's result equals cc3d_amd_connected_components3d on that volume
- * alone.  Every launch (local labelling, cross-tile unions, root scan,
- * finalize) covers all tiles of all volumes, so small volumes fill the GPU and
- * a batch costs one host synchronisation.  Supported per volume: multilabel or
- * binary (per batch), connectivity 26, 18 or 6 (8 and 4 on a single plane,
- * multilabel only), no periodic boundary, no delta.  The sum of the volumes'
- * voxels must stay below 2^32 - 32 * n. */
+ * volume's counters and labels equal cc3d_amd_label_begin/_finish's on that
+ * volume alone (default flags).  Every launch (local labelling, cross-tile
+ * unions, root scan, finalize) covers all tiles of all volumes, so small
+ * volumes fill the GPU and a batch costs one host synchronisation.  One mode
+ * per batch: multilabel, binary or continuous (one delta for all volumes).
+ * Per volume: connectivity 26, 18 or 6, or 8 and 4 on a single plane, and
+ * optionally a periodic boundary (not continuous).  Not covered: binary
+ * 2D-8 and 2D-4 volumes and continuous 2D-8 volumes (connectivity 8, or 8
+ * and 4 for binary, given for a plane).  The sum of the volumes' voxels must
+ * stay below 2^32 - 32 * n. */
 typedef struct {
   const void* in;          /* device pointer, dense, x fastest */
   int64_t sx, sy, sz;
-  int64_t connectivity;    /* | (CC3D_AMD_TILES_ROWS or _VOXELS) << 8 to choose the tile family */
+  int64_t connectivity;    /* | (CC3D_AMD_TILES_ROWS or _VOXELS) << 8 to choose the tile family,
+                              | CC3D_AMD_VOLUME_PERIODIC for a periodic boundary */
 } cc3d_amd_volume;
+
+#define CC3D_AMD_VOLUME_PERIODIC (1 << 16)
+/* mode of a batch */
+#define CC3D_AMD_BATCH_MULTILABEL 0
+#define CC3D_AMD_BATCH_BINARY 1
+#define CC3D_AMD_BATCH_CONTINUOUS 2
 
 typedef struct cc3d_amd_batch cc3d_amd_batch;
 
 /* Bytes of scratch a batch needs (0 if a volume is unsupported). */
-size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int binary_image);
+size_t cc3d_amd_batch_workspace_bytes(const cc3d_amd_volume* vols, int n, int mode);
 /* Plans a batch (no launches).  workspace may be NULL (allocated here and
  * freed by cc3d_amd_batch_free); a caller workspace must be 256-byte aligned.
  * All later launches go to `stream`. */
 int cc3d_amd_batch_create(cc3d_amd_batch** batch, const cc3d_amd_volume* vols, int n, int in_dtype,
-                          int binary_image, void* workspace, size_t workspace_bytes, void* stream);
+                          int mode, void* workspace, size_t workspace_bytes, void* stream);
+/* Continuous batches: delta (a value of in_dtype), before the first begin or
+ * graph capture. */
+int cc3d_amd_batch_set_delta(cc3d_amd_batch* batch, const void* delta);
 /* Output buffer and dtype (1/2/3) of every volume; uploaded stream-ordered. */
 int cc3d_amd_batch_set_outputs(cc3d_amd_batch* batch, void* const* outs, const int* out_dtypes);
 /* Stage 1 for all volumes (labels, counters); no host synchronisation. */
