@@ -409,7 +409,8 @@ def _tile_flags(pl, buf) -> int:
   """Tile family of the local kernel for this volume (cc3d/_tune.py)."""
   from . import _tune
   if pl.flags & L.FLAG_CONTINUOUS:
-    return 0
+    # continuous links have no row-tile kernel; tall tiles only when forced
+    return _tune._FLAG["tall"] if _tune.FORCE == "tall" else 0
   isz = pl.view_dtype.itemsize
   return _tune.tiles(pl.connectivity, pl.binary_image, isz, pl.voxels, pl.sz > 1,
                      lambda: _tune.sample_density(buf, isz, pl.voxels))

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libcc3d_amd.so")
 
 OK, ERR_RUNTIME, ERR_INVALID, ERR_HIP, ERR_NOMEM = 0, 1, 2, 3, 4
 FLAG_CONTINUOUS, FLAG_COUNT_PROVISIONAL, FLAG_NO_SINGLE_ROW_RULE = 1, 2, 4
-FLAG_TILES_ROWS, FLAG_TILES_VOXELS = 8, 16
+FLAG_TILES_ROWS, FLAG_TILES_VOXELS, FLAG_TILES_TALL = 8, 16, 32
 VOLUME_PERIODIC = 1 << 16
 BATCH_MULTILABEL, BATCH_BINARY, BATCH_CONTINUOUS = 0, 1, 2
 

@@ -1,17 +1,19 @@
 """Tile-family choice for the local labelling kernel, from a measured table.
 
-The local kernel comes in two tile families with identical results: voxel
-tiles (32x8x8, one union-find node per voxel; csrc/local.hpp) and row tiles
-(64x8x8, x-runs as nodes; csrc/rows.hpp).  Which is faster depends on the
-connectivity, the voxel type, the volume's size and how much of it is
-foreground.  `tile_table.json` (next to this file) holds, per key
+The local kernel comes in three tile families with identical results: voxel
+tiles (32x8x8, one union-find node per voxel, 256 threads; csrc/local.hpp),
+tall voxel tiles (32x16x8, 512 threads: fewer tile borders and a smaller halo
+per voxel for a larger LDS footprint) and row tiles (64x8x8, x-runs as nodes;
+csrc/rows.hpp).  Which is fastest depends on the connectivity, the voxel
+type, the volume's size and how much of it is foreground.  `tile_table.json`
+(next to this file) holds, per key
 
     connectivity / mode / voxel bytes / size class / density class
 
-the family measured faster and both timings.  `python -m cc3d._tune` measures
-every key on this GPU with synthetic volumes (Voronoi labellings, optionally
-thinned by a Bernoulli mask, and Bernoulli binary images) and rewrites the
-table.  At call time the key's size class comes from the shape; the density
+the family chosen and every family's timing.  `python -m cc3d._tune`
+measures every key on this GPU with synthetic volumes (Voronoi labellings,
+optionally thinned by a Bernoulli mask, and Bernoulli binary images) and
+rewrites the table.  At call time the key's size class comes from the shape; the density
 class is sampled from the device volume (4096 evenly spaced voxels, one
 small reduction) only when the table's choice for that key depends on it.
 Keys missing from the table keep the default family (rows for 6, voxels
@@ -34,10 +36,11 @@ TABLE_PATH = os.path.join(_HERE, "tile_table.json")
 
 SIZE_CLASSES = ((1 << 19, "s"), (1 << 23, "m"), (None, "l"))  # voxels: <= 512 Ki, <= 8 Mi, more
 DENSITY_CLASSES = ((0.3, "sparse"), (0.6, "mid"), (None, "dense"))  # foreground fraction
-_FLAG = {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS}
+FAMILIES = ("voxels", "tall", "rows")
+_FLAG = {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS, "tall": L.FLAG_TILES_TALL}
 
 _table = None
-# the tuner's own override: "rows" / "voxels" forces a family for every call
+# the tuner's own override: "rows" / "voxels" / "tall" forces a family for every call
 FORCE: Optional[str] = None
 
 
@@ -125,16 +128,20 @@ def _volume(synth, mode: str, itemsize: int, shape, density: str, seed: int) -> 
   return vol
 
 
-# The other family replaces the default one only when it is at least this much
+# Another family replaces the default one only when it is at least this much
 # faster (measured differences below it are within run-to-run noise).
 MARGIN = 0.02
 
 
-def decide(connectivity: int, ms_voxels: float, ms_rows: float) -> str:
-  default = "rows" if connectivity == 6 else "voxels"
-  other = "voxels" if default == "rows" else "rows"
-  ms = {"voxels": ms_voxels, "rows": ms_rows}
-  return other if ms[other] < (1.0 - MARGIN) * ms[default] else default
+def default_family(connectivity: int) -> str:
+  return "rows" if connectivity == 6 else "voxels"
+
+
+def decide(connectivity: int, ms: dict) -> str:
+  """The fastest family of {family: ms} if it beats the default by MARGIN."""
+  default = default_family(connectivity)
+  best = min(ms, key=lambda f: (ms[f], f != default))
+  return best if ms[best] < (1.0 - MARGIN) * ms[default] else default
 
 
 def measure(reps: int = 5, log=print) -> dict:
@@ -157,7 +164,7 @@ def measure(reps: int = 5, log=print) -> dict:
               shape, (1, shape[0], shape[0] * shape[1]))
             frac = float(np.count_nonzero(vol)) / vol.size
             ms = {}
-            for fam in ("voxels", "rows"):
+            for fam in FAMILIES:
               FORCE = fam
               try:
                 for _ in range(2):
@@ -174,11 +181,12 @@ def measure(reps: int = 5, log=print) -> dict:
                 ms[fam] = float(np.median(times))
               finally:
                 FORCE = None
-            choice = decide(conn, ms["voxels"], ms["rows"])
-            entries[k][density_class(frac)] = {"choice": choice, "ms_voxels": round(ms["voxels"], 4),
-                                               "ms_rows": round(ms["rows"], 4), "foreground": round(frac, 3)}
-            log(f"[tune {time.time() - started:6.1f}s] {k} {dens} fg={frac:.2f} voxels {ms['voxels']:.3f} ms "
-                f"rows {ms['rows']:.3f} ms -> {choice}")
+            choice = decide(conn, ms)
+            ent = {"choice": choice, "foreground": round(frac, 3)}
+            ent.update({f"ms_{f}": round(ms[f], 4) for f in FAMILIES})
+            entries[k][density_class(frac)] = ent
+            log(f"[tune {time.time() - started:6.1f}s] {k} {dens} fg={frac:.2f} "
+                + " ".join(f"{f} {ms[f]:.3f} ms" for f in FAMILIES) + f" -> {choice}")
             del d
   return entries
 

@@ -503,85 +503,115 @@ __device__ inline uint32_t root_label(const GlobalParams& p, uint32_t r, uint32_
   return p.prefix[wd] - base + __popc(p.bits[wd] & ((1u << (r & 31)) - 1u)) + 1u;
 }
 
-// Root indices of thread t's 8 J entries of tile `slot` (one vector load),
-// and the tile's label table: entry 0 background, entry 1 + i the label of
-// local root i.
+// J entries finalized per thread (a tile's entries over one workgroup), and
+// tiles per workgroup: all loads of a workgroup's tiles (root indices and
+// label tables) are issued before any store, so each thread keeps 16 entries
+// in flight.
 template <class G>
-__device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t root0, uint32_t base, uint32_t (&j)[8],
-                                uint32_t* lab) {
-  static_assert(G::TV == 8 * WG, "each thread finalizes 8 consecutive J entries");
+__host__ __device__ constexpr int fin_ept() {
+  return G::TV / WG;
+}
+template <class G>
+__host__ __device__ constexpr int fin_tiles() {
+  return fin_ept<G>() >= 16 ? 1 : 16 / fin_ept<G>();
+}
+
+// Root indices of thread t's EPT consecutive J entries of tile `slot` (vector
+// loads), and the tile's label table: entry 0 background, entry 1 + i the
+// label of local root i.
+template <class G>
+__device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t root0, uint32_t base,
+                                uint32_t (&j)[fin_ept<G>()], uint32_t* lab) {
+  constexpr int EPT = fin_ept<G>();
+  static_assert(EPT * WG == G::TV && (EPT == 8 || EPT == 16), "each thread finalizes 8 or 16 consecutive J entries");
   const int t = threadIdx.x;
   const uint32_t R = p.TR[slot];
   if (R <= 255) {
-    const uint64_t w = reinterpret_cast<const uint64_t*>(p.J8 + slot * G::TV)[t];
+    uint32_t ws[EPT / 4];
+    if constexpr (EPT == 8) {
+      const uint2 w = reinterpret_cast<const uint2*>(p.J8 + slot * G::TV)[t];
+      ws[0] = w.x;
+      ws[1] = w.y;
+    } else {
+      const uint4 w = reinterpret_cast<const uint4*>(p.J8 + slot * G::TV)[t];
+      ws[0] = w.x;
+      ws[1] = w.y;
+      ws[2] = w.z;
+      ws[3] = w.w;
+    }
 #pragma unroll
-    for (int k = 0; k < 8; k++) j[k] = (uint32_t)(w >> (8 * k)) & 0xFFu;
+    for (int k = 0; k < EPT; k++) j[k] = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
   } else {
-    const uint4 w = reinterpret_cast<const uint4*>(p.J16 + slot * G::TV)[t];
-    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    uint32_t ws[EPT / 2];
 #pragma unroll
-    for (int k = 0; k < 8; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    for (int h = 0; h < EPT / 8; h++) {
+      const uint4 w = reinterpret_cast<const uint4*>(p.J16 + slot * G::TV)[(EPT / 8) * t + h];
+      ws[4 * h] = w.x;
+      ws[4 * h + 1] = w.y;
+      ws[4 * h + 2] = w.z;
+      ws[4 * h + 3] = w.w;
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
   const uint32_t* __restrict__ LT = p.RL + root0;
   lab[0] = 0u;
   for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
 }
 
-// Thread t's 8 output voxels of tile `tile` of volume v (after a barrier
+// Thread t's EPT output voxels of tile `tile` of volume v (after a barrier
 // behind fin_load).
 template <class OUT, class G>
-__device__ inline void fin_store(const VolGeo& v, int64_t tile, OUT* __restrict__ out, const uint32_t (&j)[8],
-                                 const uint32_t* lab) {
+__device__ inline void fin_store(const VolGeo& v, int64_t tile, OUT* __restrict__ out,
+                                 const uint32_t (&j)[fin_ept<G>()], const uint32_t* lab) {
+  constexpr int EPT = fin_ept<G>();
   const int t = threadIdx.x;
   const int64_t tz = tile / (v.ntx * v.nty);
   const int64_t tr = tile - tz * v.ntx * v.nty;
   const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < EPT; k++) {
     int vx, vy, vz;
-    G::entry_voxel(8 * t + k, vx, vy, vz);
+    G::entry_voxel(EPT * t + k, vx, vy, vz);
     const int64_t gx = tx * G::TX + vx, gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
     if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[j[k]];
   }
 }
 
-// One workgroup finalizes FIN_TILES consecutive tiles: all their loads (root
-// indices and label tables) are issued before any store, so each wave keeps
-// more memory requests in flight.
-constexpr int FIN_TILES = 2;
-
 template <class OUT, class G>
 __global__ __launch_bounds__(WG) void k_finalize(GlobalParams p, OUT* __restrict__ out) {
-  __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
-  const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
-  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FIN_TILES;
+  constexpr int FT = fin_tiles<G>();
+  __shared__ uint32_t lab[FT][G::TV + 1];
+  const int64_t ngroups = (p.ntiles + FT - 1) / FT;
+  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FT;
   const VolGeo v = vol_geo(p, 0);
-  uint32_t j[FIN_TILES][8];
+  uint32_t j[FT][fin_ept<G>()];
 #pragma unroll
-  for (int n = 0; n < FIN_TILES; n++) {
+  for (int n = 0; n < FT; n++) {
     if (first + n >= p.ntiles) break;
     fin_load<G>(p, first + n, slot_root0<G>(v, first + n), 0u, j[n], lab[n]);
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < FIN_TILES; n++) {
+  for (int n = 0; n < FT; n++) {
     if (first + n >= p.ntiles) break;
     fin_store<OUT, G>(v, first + n, out, j[n], lab[n]);
   }
 }
 
-// Batch: FIN_TILES consecutive tiles per workgroup, each writing its own
+// Batch: fin_tiles consecutive tiles per workgroup, each writing its own
 // volume's output in that volume's dtype, numbered from the prefix at the
 // volume's first key word.
 template <class G>
 __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
-  __shared__ uint32_t lab[FIN_TILES][G::TV + 1];
-  const int64_t ngroups = (p.ntiles + FIN_TILES - 1) / FIN_TILES;
-  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FIN_TILES;
-  uint32_t j[FIN_TILES][8];
-  int vi[FIN_TILES];
+  constexpr int FT = fin_tiles<G>();
+  __shared__ uint32_t lab[FT][G::TV + 1];
+  const int64_t ngroups = (p.ntiles + FT - 1) / FT;
+  const int64_t first = xcd_tile(blockIdx.x, ngroups) * FT;
+  uint32_t j[FT][fin_ept<G>()];
+  int vi[FT];
 #pragma unroll
-  for (int n = 0; n < FIN_TILES; n++) {
+  for (int n = 0; n < FT; n++) {
     const int64_t slot = first + n;
     if (slot >= p.ntiles) break;
     vi[n] = (int)p.vix[slot];
@@ -590,7 +620,7 @@ __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < FIN_TILES; n++) {
+  for (int n = 0; n < FT; n++) {
     const int64_t slot = first + n;
     if (slot >= p.ntiles) break;
     const BatchVol& bv = p.vols[vi[n]];
@@ -1135,6 +1165,7 @@ struct Geometry {
   int nb = 0, qk = Q_NONE, mode = M_MULTI;
   bool is3d = true;
   bool rows = false;  // row tiles (GeoR): 3D multilabel and binary
+  bool tall = false;  // tall voxel tiles (Geo3T) instead of Geo3
   int64_t ntx = 0, nty = 0, ntz = 0, ntiles = 0;
   int TV = 0, HALO = 0;
   int64_t nwords = 0, nblk = 0;
@@ -1142,7 +1173,7 @@ struct Geometry {
   size_t oGP, oRL, oTR, oTQ, oTC, oJ8, oJ16, oREQ, oBITS, oPRE, oBSUM, oCTR, oMM, oQB, oQP, oQS, total;
 };
 
-// tiles: CC3D_AMD_TILES_ROWS / _VOXELS bits (else the default family)
+// tiles: CC3D_AMD_TILES_ROWS / _VOXELS / _TALL bits (else the default family)
 static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connectivity, int binary,
                 int continuous, int tiles = 0) {
   if (sx <= 0 || sy <= 0 || sz <= 0) return fail(CC3D_AMD_ERR_INVALID, "volume must be non-empty");
@@ -1174,9 +1205,14 @@ static int plan(Geometry& g, int64_t sx, int64_t sy, int64_t sz, int64_t connect
   // Row tiles for 6-connected 3D volumes (multilabel or binary): measured
   // faster there than the voxel tiles; the voxel tiles stay faster at 26/18.
   g.rows = g.is3d && g.mode != M_CONT &&
-           ((tiles & CC3D_AMD_TILES_ROWS) ? true : (tiles & CC3D_AMD_TILES_VOXELS) ? false : connectivity == 6);
+           ((tiles & CC3D_AMD_TILES_ROWS)                            ? true
+            : (tiles & (CC3D_AMD_TILES_VOXELS | CC3D_AMD_TILES_TALL)) ? false
+                                                                      : connectivity == 6);
+  g.tall = g.is3d && !g.rows && (tiles & CC3D_AMD_TILES_TALL) && !(tiles & CC3D_AMD_TILES_VOXELS);
   if (g.rows) {
     TX = GeoR::TX; TY = GeoR::TY; TZ = GeoR::TZ; g.TV = GeoR::TV; g.HALO = GeoR::HALO;
+  } else if (g.tall) {
+    TX = Geo3T::TX; TY = Geo3T::TY; TZ = Geo3T::TZ; g.TV = Geo3T::TV; g.HALO = Geo3T::HALO;
   } else if (g.is3d) {
     TX = Geo3::TX; TY = Geo3::TY; TZ = Geo3::TZ; g.TV = Geo3::TV; g.HALO = Geo3::HALO;
   } else {
@@ -1255,7 +1291,7 @@ struct cc3d_amd_batch {
   int64_t ntiles = 0, nkeys = 0, nwords = 0, nblk = 0;
   size_t oVOL = 0, oGP = 0, oRL = 0, oBITS = 0, oPRE = 0, oBSUM = 0, oCTR = 0;
   BatchVol* h_tab = nullptr;  // pinned host copy of the volume table
-  int64_t* h_t0 = nullptr;    // pinned: first tile of each volume, per tile group (n + 2)
+  int64_t* h_t0 = nullptr;    // pinned: first tile of each volume, per tile group (n + tile groups)
   Counters* h_ctr = nullptr;  // pinned: entry 0 whole batch, 1 + k table entry k
   // The table holds the volumes grouped by connectivity (one local launch
   // per group, 26 and 18 first); perm[k] is the caller's index of table
@@ -1267,8 +1303,11 @@ struct cc3d_amd_batch {
     int nb, tg;
     int64_t tile0, tiles;  // within tile group tg
   };
+  // tile families of a batch (one geometry and set of per-tile arrays each)
+  enum { FAM_VOXELS = 0, FAM_TALL = 1, FAM_ROWS = 2 };
   struct TileGroup {
-    bool rows = false;
+    int fam = FAM_VOXELS;
+    bool rows() const { return fam == FAM_ROWS; }
     int v0 = 0, v1 = 0;  // table entries
     int64_t tiles = 0;
     size_t oT0 = 0, oVIX = 0, oTR = 0, oTQ = 0, oTC = 0, oJ8 = 0, oJ16 = 0, oREQ = 0;
@@ -1595,9 +1634,9 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
   ProfScope ps_("k_local", s);
   if (g.rows && g.mode == M_MULTI) e = launch_rows_multi(in_dtype, g.nb, lp, s);
   else if (g.rows) e = launch_rows_binary(in_dtype, g.nb, lp, s);
-  else if (g.mode == M_MULTI) e = launch_local_multi(in_dtype, g.nb, g.qk, lp, s);
-  else if (g.mode == M_BINARY) e = launch_local_binary(in_dtype, g.nb, g.qk, lp, s);
-  else e = launch_local_cont(in_dtype, g.nb, g.qk, lp, s);
+  else if (g.mode == M_MULTI) e = launch_local_multi(in_dtype, g.nb, g.qk, g.tall, lp, s);
+  else if (g.mode == M_BINARY) e = launch_local_binary(in_dtype, g.nb, g.qk, g.tall, lp, s);
+  else e = launch_local_cont(in_dtype, g.nb, g.qk, g.tall, lp, s);
   }
   if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_local: ") + hipGetErrorString(e));
   {
@@ -1606,7 +1645,10 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
                        gp.ctr);
   }
   if (hipGetLastError() != hipSuccess) return fail(CC3D_AMD_ERR_HIP, "k_tile_counters launch failed");
-  rc = g.rows ? run_global<GeoR>(j) : g.is3d ? run_global<Geo3>(j) : run_global<Geo2>(j);
+  rc = g.rows ? run_global<GeoR>(j)
+       : g.tall ? run_global<Geo3T>(j)
+       : g.is3d ? run_global<Geo3>(j)
+                : run_global<Geo2>(j);
   if (rc == CC3D_AMD_OK && (j->count_prov || j->stale)) {
     switch (in_dtype) {
       case DT_U8: rc = launch_q1<uint8_t>(j); break;
@@ -1657,14 +1699,17 @@ int cc3d_amd_label_finish(cc3d_amd_job* j, void* out, int out_dtype) {
   if (!j || !out) return fail(CC3D_AMD_ERR_INVALID, "NULL job or output");
   const Geometry& g = j->g;
   hipStream_t s = j->stream;
-  const dim3 grid((unsigned)((g.ntiles + FIN_TILES - 1) / FIN_TILES)), block(WG);
+  const dim3 grid3((unsigned)((g.ntiles + fin_tiles<Geo3>() - 1) / fin_tiles<Geo3>()));
+  const dim3 grid3t((unsigned)((g.ntiles + fin_tiles<Geo3T>() - 1) / fin_tiles<Geo3T>()));
+  const dim3 grid2((unsigned)((g.ntiles + fin_tiles<Geo2>() - 1) / fin_tiles<Geo2>())), block(WG);
   int rc = CC3D_AMD_OK;
   ProfScope ps_("k_finalize", s);
 #define CC3D_F(OUT)                                                                                \
   do {                                                                                             \
     if (g.rows) hipLaunchKernelGGL((k_rfinalize<OUT>), dim3((unsigned)g.ntiles), block, 0, s, j->gp, (OUT*)out); \
-    else if (g.is3d) hipLaunchKernelGGL((k_finalize<OUT, Geo3>), grid, block, 0, s, j->gp, (OUT*)out); \
-    else hipLaunchKernelGGL((k_finalize<OUT, Geo2>), grid, block, 0, s, j->gp, (OUT*)out);        \
+    else if (g.tall) hipLaunchKernelGGL((k_finalize<OUT, Geo3T>), grid3t, block, 0, s, j->gp, (OUT*)out); \
+    else if (g.is3d) hipLaunchKernelGGL((k_finalize<OUT, Geo3>), grid3, block, 0, s, j->gp, (OUT*)out); \
+    else hipLaunchKernelGGL((k_finalize<OUT, Geo2>), grid2, block, 0, s, j->gp, (OUT*)out);        \
     if (j->stale) {                                                                                \
       switch (j->dt) {                                                                             \
         case DT_U8: rc = launch_stale<uint8_t, OUT>(j, (OUT*)out); break;                          \
@@ -1886,17 +1931,26 @@ int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* c
 // 4-neighbourhood, so 8 and 4 map to 26 and 6.  Not covered: binary 2D-8 and
 // 2D-4 (their block and first-row numberings) and continuous 2D-8 (its global
 // min/max rule).
+static void batch_tile_dims(int fam, int& TX, int& TY, int& TZ) {
+  if (fam == cc3d_amd_batch::FAM_ROWS) {
+    TX = GeoR::TX; TY = GeoR::TY; TZ = GeoR::TZ;
+  } else if (fam == cc3d_amd_batch::FAM_TALL) {
+    TX = Geo3T::TX; TY = Geo3T::TY; TZ = Geo3T::TZ;
+  } else {
+    TX = Geo3::TX; TY = Geo3::TY; TZ = Geo3::TZ;
+  }
+}
+
 static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vector<BatchVol>& tab,
                         std::vector<int64_t>& t0, cc3d_amd_batch& b) {
   if (!vols || n <= 0) return fail(CC3D_AMD_ERR_INVALID, "a batch needs at least one volume");
   if (mode != CC3D_AMD_BATCH_MULTILABEL && mode != CC3D_AMD_BATCH_BINARY && mode != CC3D_AMD_BATCH_CONTINUOUS)
     return fail(CC3D_AMD_ERR_INVALID, "batch mode must be multilabel, binary or continuous");
   tab.resize(n);
-  t0.resize(n + 2);
   b.perm.resize(n);
   b.groups.clear();
   std::vector<int> nbs(n), nbps(n);
-  std::vector<char> rowsv(n);
+  std::vector<int> famv(n);
   for (int i = 0; i < n; i++) {  // checked in the caller's order
     const cc3d_amd_volume& v = vols[i];
     if (!v.in) return fail(CC3D_AMD_ERR_INVALID, "NULL input in batch");
@@ -1904,7 +1958,8 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
     const int conn = (int)(v.connectivity & 0xFF);
     int nb = conn;
     if ((v.connectivity &
-         ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS) << 8 | CC3D_AMD_VOLUME_PERIODIC | 0xFF)) != 0)
+         ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS | CC3D_AMD_TILES_TALL) << 8 |
+                    CC3D_AMD_VOLUME_PERIODIC | 0xFF)) != 0)
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     if (nb == 8 || nb == 4) {
       if (v.sz != 1) return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
@@ -1924,31 +1979,35 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
       nbps[i] = v.sz > 1 ? 6 : (conn == 26 || conn == 18) ? 8 : conn == 6 ? 4 : conn;
     }
     nbs[i] = nb;
+    // the single-volume path's choice (plan): rows unless continuous, then
+    // voxels, then tall; by default rows at 6-connectivity, else voxels
     const int pref = (int)((v.connectivity >> 8) & 0xFF);
-    rowsv[i] = mode == CC3D_AMD_BATCH_CONTINUOUS ? false
-               : (pref & CC3D_AMD_TILES_ROWS)    ? true
-               : (pref & CC3D_AMD_TILES_VOXELS)  ? false
-                                                 : nb == 6;
+    const bool cont = mode == CC3D_AMD_BATCH_CONTINUOUS;
+    famv[i] = (!cont && (pref & CC3D_AMD_TILES_ROWS))  ? cc3d_amd_batch::FAM_ROWS
+              : (pref & CC3D_AMD_TILES_VOXELS)          ? cc3d_amd_batch::FAM_VOXELS
+              : (pref & CC3D_AMD_TILES_TALL)            ? cc3d_amd_batch::FAM_TALL
+              : (!cont && nb == 6)                      ? cc3d_amd_batch::FAM_ROWS
+                                                        : cc3d_amd_batch::FAM_VOXELS;
     b.perm[i] = i;
   }
-  // voxel-tile volumes first, then row-tile ones; within each, 26, 18, 6
+  // by tile family (voxels, tall, rows), then 26, 18, 6 within each
   std::stable_sort(b.perm.begin(), b.perm.end(), [&](int x, int y) {
-    return rowsv[x] != rowsv[y] ? !rowsv[x] : nbs[x] > nbs[y];
+    return famv[x] != famv[y] ? famv[x] < famv[y] : nbs[x] > nbs[y];
   });
   uint64_t key = 0;
   b.tgroups.clear();
   b.periodic.clear();
   for (int k = 0; k < n; k++) {
-    const int i = b.perm[k], nb = nbs[i];
-    const bool rows = rowsv[i] != 0;
-    if (b.tgroups.empty() || b.tgroups.back().rows != rows) {
+    const int i = b.perm[k], nb = nbs[i], fam = famv[i];
+    if (b.tgroups.empty() || b.tgroups.back().fam != fam) {
       cc3d_amd_batch::TileGroup tg;
-      tg.rows = rows;
+      tg.fam = fam;
       tg.v0 = k;
       b.tgroups.push_back(tg);
     }
     cc3d_amd_batch::TileGroup& tg = b.tgroups.back();
-    const int TXg = rows ? GeoR::TX : Geo3::TX, TYg = rows ? GeoR::TY : Geo3::TY, TZg = rows ? GeoR::TZ : Geo3::TZ;
+    int TXg, TYg, TZg;
+    batch_tile_dims(fam, TXg, TYg, TZg);
     const cc3d_amd_volume& v = vols[i];
     BatchVol& t = tab[k];
     t.in = v.in;
@@ -1978,6 +2037,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
   }
   // first tile of each volume within its tile group; tile group g's entries
   // start at index v0 + g (one closing entry per group)
+  t0.assign(n + b.tgroups.size(), 0);
   for (size_t g = 0; g < b.tgroups.size(); g++) {
     const auto& tg = b.tgroups[g];
     for (int k = tg.v0; k < tg.v1; k++) t0[k + g] = tab[k].tile0;
@@ -2003,7 +2063,10 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
   b.oBSUM = take((size_t)(b.nblk + 1) * 4);
   b.oCTR = take((size_t)(n + 1) * sizeof(Counters));
   for (auto& tg : b.tgroups) {
-    const size_t TVg = tg.rows ? GeoR::TV : Geo3::TV, HALOg = tg.rows ? GeoR::HALO : Geo3::HALO;
+    int TXg, TYg, TZg;
+    batch_tile_dims(tg.fam, TXg, TYg, TZg);
+    const size_t TVg = (size_t)TXg * TYg * TZg;
+    const size_t HALOg = tg.rows() ? GeoR::HALO : tg.fam == cc3d_amd_batch::FAM_TALL ? Geo3T::HALO : Geo3::HALO;
     const size_t slots = (size_t)tg.tiles * TVg;
     tg.oT0 = take((size_t)(tg.v1 - tg.v0 + 1) * sizeof(int64_t));
     tg.oVIX = take((size_t)tg.tiles * 4);
@@ -2062,14 +2125,15 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     b->own_ws = true;
   }
   if (hipHostMalloc(reinterpret_cast<void**>(&b->h_tab), n * sizeof(BatchVol), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&b->h_t0), (n + 2) * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_t0), t0.size() * sizeof(int64_t), hipHostMallocDefault) !=
+        hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&b->h_ctr), (n + 1) * sizeof(Counters), hipHostMallocDefault) !=
         hipSuccess) {
     cc3d_amd_batch_free(b);
     return fail(CC3D_AMD_ERR_NOMEM, "could not allocate pinned batch tables");
   }
   std::memcpy(b->h_tab, tab.data(), n * sizeof(BatchVol));
-  std::memcpy(b->h_t0, t0.data(), (n + 2) * sizeof(int64_t));
+  std::memcpy(b->h_t0, t0.data(), t0.size() * sizeof(int64_t));
   char* ws = b->ws;
   if (hipMemcpyAsync(ws + b->oVOL, b->h_tab, n * sizeof(BatchVol), hipMemcpyHostToDevice, b->stream) != hipSuccess) {
     cc3d_amd_batch_free(b);
@@ -2126,7 +2190,7 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     lp.REQ = reinterpret_cast<uint32_t*>(ws + tg.oREQ);
     lp.TQ = reinterpret_cast<uint32_t*>(ws + tg.oTQ);
     lp.tcnt = reinterpret_cast<TileCnt*>(ws + tg.oTC);
-    need_utab = need_utab || !tg.rows;
+    need_utab = need_utab || !tg.rows();
   }
   if (need_utab) {
     const uint16_t* ut = nullptr;
@@ -2187,16 +2251,17 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     ProfScope ps_("k_local", s);
     for (const auto& g : b->groups) {
       const auto& tg = b->tgroups[g.tg];
-      if (tg.rows && b->mode == M_BINARY)
+      const bool tall = tg.fam == cc3d_amd_batch::FAM_TALL;
+      if (tg.rows() && b->mode == M_BINARY)
         e = launch_rows_batch_binary(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
-      else if (tg.rows)
+      else if (tg.rows())
         e = launch_rows_batch_multi(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else if (b->mode == M_BINARY)
-        e = launch_local_batch_binary(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+        e = launch_local_batch_binary(b->dt, g.nb, tall, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else if (b->mode == M_CONT)
-        e = launch_local_batch_cont(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+        e = launch_local_batch_cont(b->dt, g.nb, tall, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else
-        e = launch_local_batch_multi(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+        e = launch_local_batch_multi(b->dt, g.nb, tall, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       if (e != hipSuccess) break;
     }
   }
@@ -2212,7 +2277,9 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     ProfScope ps_("k_requests", s);
     for (const auto& tg : b->tgroups) {
       const unsigned blocks = grid_for(tg.tiles, WG / TILE_LANES, 65536);
-      if (tg.rows) hipLaunchKernelGGL((k_requests<GeoR, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      if (tg.rows()) hipLaunchKernelGGL((k_requests<GeoR, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      else if (tg.fam == cc3d_amd_batch::FAM_TALL)
+        hipLaunchKernelGGL((k_requests<Geo3T, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
     }
   }
@@ -2220,7 +2287,10 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   if (!b->periodic.empty()) {
     ProfScope ps_("k_periodic", s);
     for (const auto& pv : b->periodic) {
-      e = b->tgroups[pv.tg].rows ? batch_periodic<GeoR>(b, pv, s) : batch_periodic<Geo3>(b, pv, s);
+      const int fam = b->tgroups[pv.tg].fam;
+      e = fam == cc3d_amd_batch::FAM_ROWS   ? batch_periodic<GeoR>(b, pv, s)
+          : fam == cc3d_amd_batch::FAM_TALL ? batch_periodic<Geo3T>(b, pv, s)
+                                            : batch_periodic<Geo3>(b, pv, s);
       if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_periodic: ") + hipGetErrorString(e));
     }
   }
@@ -2228,7 +2298,9 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     ProfScope ps_("k_roots", s);
     for (const auto& tg : b->tgroups) {
       const unsigned blocks = grid_for(tg.tiles, WG / TILE_LANES, 65536);
-      if (tg.rows) hipLaunchKernelGGL((k_roots<GeoR>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      if (tg.rows()) hipLaunchKernelGGL((k_roots<GeoR>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      else if (tg.fam == cc3d_amd_batch::FAM_TALL)
+        hipLaunchKernelGGL((k_roots<Geo3T>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else hipLaunchKernelGGL((k_roots<Geo3>), dim3(blocks), dim3(WG), 0, s, tg.gp);
     }
   }
@@ -2268,11 +2340,14 @@ int cc3d_amd_batch_finish(cc3d_amd_batch* b) {
   if (!b->outputs_set) return fail(CC3D_AMD_ERR_INVALID, "batch outputs are not set");
   ProfScope ps_("k_finalize", b->stream);
   for (const auto& tg : b->tgroups) {
-    if (tg.rows)
+    if (tg.rows())
       hipLaunchKernelGGL(k_rfinalize_batch, dim3((unsigned)tg.tiles), dim3(WG), 0, b->stream, tg.gp);
+    else if (tg.fam == cc3d_amd_batch::FAM_TALL)
+      hipLaunchKernelGGL((k_finalize_batch<Geo3T>), dim3((unsigned)((tg.tiles + fin_tiles<Geo3T>() - 1) / fin_tiles<Geo3T>())),
+                         dim3(WG), 0, b->stream, tg.gp);
     else
-      hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)((tg.tiles + FIN_TILES - 1) / FIN_TILES)), dim3(WG),
-                         0, b->stream, tg.gp);
+      hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)((tg.tiles + fin_tiles<Geo3>() - 1) / fin_tiles<Geo3>())),
+                         dim3(WG), 0, b->stream, tg.gp);
   }
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;

@@ -232,6 +232,10 @@ __host__ __device__ inline uint32_t tile_root0(int64_t sx, int64_t sy, int64_t s
 }
 
 using Geo3 = TileGeo<32, 8, 8, true, 256>;
+// Tall voxel tiles: twice the rows, 512 threads.  Fewer tile borders (18 % of
+// the columns instead of 30 %) and a smaller halo per voxel (34 % instead of
+// 49 %), for a larger LDS footprint per workgroup.
+using Geo3T = TileGeo<32, 16, 8, true, 512>;
 using Geo2 = TileGeo<64, 32, 1, false, 256>;
 
 // Global key of a voxel.

@@ -456,7 +456,8 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   constexpr int CSTEP = IS3D ? G::BX * G::BY : G::BX;  // box step along the column
   constexpr int RW = LocalLds<T, G, FULLQ>::RW;
   static_assert(HPT <= 32 && (!G::IS3D || (G::TZ <= 32 && G::BY <= 32)), "halo piece flags fit one word");
-  static_assert(RW <= 128, "one wave scans the root bitmask");
+  constexpr int WPL = (RW + 63) / 64;  // root bitmask words per lane of the scanning wave
+  static_assert(WPL <= 4, "one wave scans the root bitmask");
   // 3D transitive 26/18: halo staged as runs (see step 1)
   constexpr bool HRUNS = PIECES && IS3D && MODE != M_CONT && (NB == 26 || NB == 18);
   // 26-connected border run members consult the ring columns' run breaks (step 3)
@@ -1067,18 +1068,26 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
 
   // 6. rank the marked roots in node (= key) order: one wave scans the bitmask
   if (wv == 0) {
-    const int w0 = 2 * lane;
-    const uint32_t c0 = w0 < RW ? (uint32_t)__popc(L.w.rbits[w0]) : 0u;
-    const uint32_t c1 = w0 + 1 < RW ? (uint32_t)__popc(L.w.rbits[w0 + 1]) : 0u;
-    uint32_t incl = c0 + c1;
+    const int w0 = WPL * lane;
+    uint32_t c[WPL];
+    uint32_t own = 0;
+#pragma unroll
+    for (int i = 0; i < WPL; i++) {
+      c[i] = w0 + i < RW ? (uint32_t)__popc(L.w.rbits[w0 + i]) : 0u;
+      own += c[i];
+    }
+    uint32_t incl = own;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(incl, o);
       if (lane >= o) incl += y;
     }
-    const uint32_t ex = incl - c0 - c1;
-    if (w0 < RW) L.late.rprefix[w0] = ex;
-    if (w0 + 1 < RW) L.late.rprefix[w0 + 1] = ex + c0;
+    uint32_t ex = incl - own;
+#pragma unroll
+    for (int i = 0; i < WPL; i++) {
+      if (w0 + i < RW) L.late.rprefix[w0 + i] = ex;
+      ex += c[i];
+    }
     if (lane == 63) L.late.nroots = incl;
   }
   __syncthreads();
@@ -1233,24 +1242,25 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>()
 }
 
 // Launchers (one translation unit per mode keeps compile times short).
-hipError_t launch_local_multi(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
-hipError_t launch_local_binary(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
-hipError_t launch_local_cont(int dt, int nb, int qk, const LocalParams& p, hipStream_t s);
+// tall: 3D tiles of Geo3T instead of Geo3.
+hipError_t launch_local_multi(int dt, int nb, int qk, bool tall, const LocalParams& p, hipStream_t s);
+hipError_t launch_local_binary(int dt, int nb, int qk, bool tall, const LocalParams& p, hipStream_t s);
+hipError_t launch_local_cont(int dt, int nb, int qk, bool tall, const LocalParams& p, hipStream_t s);
 // A batch launches once per connectivity: tiles [gtile0, gtile0 + gtiles) all
 // belong to volumes of connectivity nb.
-hipError_t launch_local_batch_multi(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
-                                    int64_t gtile0, int64_t gtiles, hipStream_t s);
-hipError_t launch_local_batch_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
-                                     int64_t gtile0, int64_t gtiles, hipStream_t s);
-hipError_t launch_local_batch_cont(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
-                                   int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_multi(int dt, int nb, bool tall, const LocalParams& p, const BatchVol* vols,
+                                    const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_binary(int dt, int nb, bool tall, const LocalParams& p, const BatchVol* vols,
+                                     const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_cont(int dt, int nb, bool tall, const LocalParams& p, const BatchVol* vols,
+                                   const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
 
-template <int MODE>
-hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
-                                   int64_t gtile0, int64_t gtiles, hipStream_t s) {
-  const dim3 grid((unsigned)gtiles), block(Geo3::NT);
+template <int MODE, class G>
+hipError_t launch_local_batch_geo(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,
+                                  int64_t gtile0, int64_t gtiles, hipStream_t s) {
+  const dim3 grid((unsigned)gtiles), block(G::NT);
 #define CC3D_B(T, NB) \
-  hipLaunchKernelGGL((k_local_batch<T, MODE, NB, Geo3>), grid, block, 0, s, p, vols, vix, gtile0, gtiles)
+  hipLaunchKernelGGL((k_local_batch<T, MODE, NB, G>), grid, block, 0, s, p, vols, vix, gtile0, gtiles)
 #define CC3D_BNB(T)                          \
   switch (nb) {                              \
     case 26: CC3D_B(T, 26); break;           \
@@ -1271,17 +1281,23 @@ hipError_t launch_local_batch_mode(int dt, int nb, const LocalParams& p, const B
 #undef CC3D_B
   return hipGetLastError();
 }
-
 template <int MODE>
-hipError_t launch_local_mode(int dt, int nb, int qk, const LocalParams& p, hipStream_t s) {
+hipError_t launch_local_batch_mode(int dt, int nb, bool tall, const LocalParams& p, const BatchVol* vols,
+                                   const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s) {
+  return tall ? launch_local_batch_geo<MODE, Geo3T>(dt, nb, p, vols, vix, gtile0, gtiles, s)
+              : launch_local_batch_geo<MODE, Geo3>(dt, nb, p, vols, vix, gtile0, gtiles, s);
+}
+
+template <int MODE, class G3>
+hipError_t launch_local_geo(int dt, int nb, int qk, const LocalParams& p, hipStream_t s) {
   const dim3 grid((unsigned)p.ntiles);
 #define CC3D_L(T, NB, QK, G) \
   hipLaunchKernelGGL((k_local<T, MODE, NB, QK, G>), grid, dim3(G::NT), 0, s, p)
 #define CC3D_NB(T)                                                               \
   switch (nb) {                                                                  \
-    case 26: CC3D_L(T, 26, Q_NONE, Geo3); break;                                 \
-    case 18: CC3D_L(T, 18, Q_NONE, Geo3); break;                                 \
-    case 6: CC3D_L(T, 6, Q_NONE, Geo3); break;                                   \
+    case 26: CC3D_L(T, 26, Q_NONE, G3); break;                                   \
+    case 18: CC3D_L(T, 18, Q_NONE, G3); break;                                   \
+    case 6: CC3D_L(T, 6, Q_NONE, G3); break;                                     \
     case 8:                                                                      \
       if (MODE == M_BINARY && qk == Q_BLOCK8) CC3D_L(T, 8, (MODE == M_BINARY ? Q_BLOCK8 : Q_NONE), Geo2); \
       else if (MODE == M_CONT && qk == Q_CONT8) CC3D_L(T, 8, (MODE == M_CONT ? Q_CONT8 : Q_NONE), Geo2); \
@@ -1305,6 +1321,10 @@ hipError_t launch_local_mode(int dt, int nb, int qk, const LocalParams& p, hipSt
 #undef CC3D_NB
 #undef CC3D_L
   return hipGetLastError();
+}
+template <int MODE>
+hipError_t launch_local_mode(int dt, int nb, int qk, bool tall, const LocalParams& p, hipStream_t s) {
+  return tall ? launch_local_geo<MODE, Geo3T>(dt, nb, qk, p, s) : launch_local_geo<MODE, Geo3>(dt, nb, qk, p, s);
 }
 
 }  // namespace cc3d_amd

@@ -109,10 +109,12 @@ typedef struct {
 #define CC3D_AMD_NO_SINGLE_ROW_RULE 4  /* do not mimic the binding's single-row shortcut */
 /* Tile family of the local labelling kernel for 3D volumes (no reference
  * counterpart: a performance choice, results are identical): row tiles
- * (64x8x8, x-runs as union-find nodes) or voxel tiles (32x8x8, one node per
- * voxel).  Neither: row tiles for 6-connectivity, voxel tiles otherwise. */
+ * (64x8x8, x-runs as union-find nodes), voxel tiles (32x8x8, one node per
+ * voxel) or tall voxel tiles (32x16x8, one node per voxel, 512 threads).
+ * None: row tiles for 6-connectivity, voxel tiles otherwise. */
 #define CC3D_AMD_TILES_ROWS 8
 #define CC3D_AMD_TILES_VOXELS 16
+#define CC3D_AMD_TILES_TALL 32
 
 /* Bytes of scratch a job of this geometry needs (0 on bad arguments). */
 size_t cc3d_amd_workspace_bytes(int64_t sx, int64_t sy, int64_t sz, int64_t connectivity,
@@ -158,7 +160,7 @@ uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t 
 typedef struct {
   const void* in;          /* device pointer, dense, x fastest */
   int64_t sx, sy, sz;
-  int64_t connectivity;    /* | (CC3D_AMD_TILES_ROWS or _VOXELS) << 8 to choose the tile family,
+  int64_t connectivity;    /* | (CC3D_AMD_TILES_ROWS, _VOXELS or _TALL) << 8 to choose the tile family,
                               | CC3D_AMD_VOLUME_PERIODIC for a periodic boundary */
 } cc3d_amd_volume;
 

@@ -75,3 +75,30 @@ def test_argument_checks_without_device_work():
   assert s(p, 2, 4, 4, 4, 3, 1, 3, None, p, None, None, None) == _lib.ERR_INVALID  # counts missing
   assert s(p, 2, 4, 4, 4, 3, 1, 3, p, p, p, None, None) == _lib.ERR_INVALID  # centroids need sums
   assert s(p, 4, 4, 4, 4, 3, 1, 3, p, p, None, None, None) == _lib.ERR_INVALID  # float labels
+
+
+def test_batch_layout_with_every_tile_family():
+  """cc3d_amd_batch_workspace_bytes plans a batch on the host alone: volumes
+  of all three tile families, modes and periodic flags lay out, and the
+  checks reject what a batch does not cover."""
+  from cc3d import _lib as L
+  lib = L.lib()
+  fams = [L.FLAG_TILES_VOXELS, L.FLAG_TILES_TALL, L.FLAG_TILES_ROWS]
+  n = 9
+  vt = (L.Volume * n)()
+  for k in range(n):
+    vt[k].inp = 0x1000 * (k + 1)  # never dereferenced: no device work
+    vt[k].sx, vt[k].sy, vt[k].sz = 40 + 7 * k, 30 + k, 20 + 3 * k
+    vt[k].connectivity = (26, 18, 6)[k % 3] | (fams[(k // 3) % 3] << 8)
+  full = lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_MULTILABEL)
+  assert full > 0
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n - 3, L.BATCH_MULTILABEL) < full
+  vt[2].connectivity |= L.VOLUME_PERIODIC
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_BINARY) > 0
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_CONTINUOUS) == 0  # periodic + continuous
+  vt[2].connectivity = 8
+  vt[2].sz = 1
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_MULTILABEL) > 0
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_BINARY) == 0  # binary 2D-8
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_CONTINUOUS) == 0  # continuous 2D-8
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, 3) == 0  # no such mode

@@ -479,6 +479,19 @@ def test_oracle_each_known_answer():
   ka.check_each(each, side=24)
 
 
+def test_tile_family_rule():
+  """The tuner's rule: the default family (rows at 6, voxels otherwise) unless
+  another is at least MARGIN faster; ties keep the default."""
+  from cc3d import _tune
+  m = _tune.MARGIN
+  assert _tune.decide(26, {"voxels": 1.0, "tall": 1.0 - m / 2, "rows": 2.0}) == "voxels"
+  assert _tune.decide(26, {"voxels": 1.0, "tall": 1.0 - 2 * m, "rows": 2.0}) == "tall"
+  assert _tune.decide(26, {"voxels": 1.0, "tall": 0.9, "rows": 0.8}) == "rows"
+  assert _tune.decide(6, {"voxels": 0.9, "tall": 0.95, "rows": 1.0}) == "voxels"
+  assert _tune.decide(6, {"voxels": 1.0, "tall": 1.0, "rows": 1.0}) == "rows"
+  assert _tune.decide(18, {"voxels": 1.0, "rows": 1.5}) == "voxels"
+
+
 def test_tile_table_lookup():
   """The tile-family table: every key present with both timings, choices
   following its rule, and the dispatcher's lookup (density sampled only
@@ -496,7 +509,9 @@ def test_tile_table_lookup():
         for cls in ("s", "m", "l"):
           row = ent[f"{conn}/{mode}/{isz}/{cls}"]
           for e in row.values():
-            assert e["choice"] == _tune.decide(conn, e["ms_voxels"], e["ms_rows"])
+            ms = {f: e[f"ms_{f}"] for f in _tune.FAMILIES if f"ms_{f}" in e}
+            assert {"voxels", "rows"} <= set(ms)
+            assert e["choice"] == _tune.decide(conn, ms)
   calls = []
 
   def density():
@@ -506,7 +521,7 @@ def test_tile_table_lookup():
   row = ent[_tune.key(6, False, 4, 400 ** 3)]
   exp = row[_tune.density_class(0.1)]["choice"] if len({v["choice"] for v in row.values()}) > 1 else \
     next(iter(row.values()))["choice"]
-  assert flag == {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS}[exp]
+  assert flag == {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS, "tall": L.FLAG_TILES_TALL}[exp]
   assert bool(calls) == (len({v["choice"] for v in row.values()}) > 1)
   assert _tune.tiles(26, False, 4, 100, False, density) == 0  # a single plane: no choice
   g = torch.Generator().manual_seed(0)
