@@ -12,12 +12,14 @@ type, the volume's size and how much of it is foreground.  `tile_table.json`
 
 the family chosen and every family's timing.  `python -m cc3d._tune`
 measures every key on this GPU with synthetic volumes (Voronoi labellings,
-optionally thinned by a Bernoulli mask, and Bernoulli binary images) and
-rewrites the table.  At call time the key's size class comes from the shape; the density
-class is sampled from the device volume (4096 evenly spaced voxels, one
-small reduction) only when the table's choice for that key depends on it.
-Keys missing from the table keep the default family (rows for 6, voxels
-otherwise).
+optionally thinned by a Bernoulli mask, and Bernoulli binary images), and the
+cost of one density sample, and rewrites the table.  At call time the key's
+size class comes from the shape.  The density class is sampled from the
+device volume (4096 evenly spaced voxels, one small reduction and a host
+synchronisation) only when the table's choice for that key depends on it and
+the best single family for the key would lose more than one sample costs on
+some density class; otherwise that family is used unsampled.  Keys missing
+from the table keep the default family (rows for 6, voxels otherwise).
 """
 from __future__ import annotations
 
@@ -40,6 +42,9 @@ FAMILIES = ("voxels", "tall", "rows")
 _FLAG = {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS, "tall": L.FLAG_TILES_TALL}
 
 _table = None
+_doc = None
+# measured cost of one density sample (ms), when the table does not hold it
+SAMPLE_MS_DEFAULT = 0.03
 # the tuner's own override: "rows" / "voxels" / "tall" forces a family for every call
 FORCE: Optional[str] = None
 
@@ -59,14 +64,30 @@ def density_class(frac: float) -> str:
 
 
 def table() -> dict:
-  global _table
+  global _table, _doc
   if _table is None:
     try:
       with open(TABLE_PATH) as f:
-        _table = json.load(f).get("entries", {})
+        _doc = json.load(f)
     except (OSError, ValueError):
-      _table = {}
+      _doc = {}
+    _table = _doc.get("entries", {})
   return _table
+
+
+def sample_ms() -> float:
+  table()
+  return float(_doc.get("sample_ms", SAMPLE_MS_DEFAULT))
+
+
+def unsampled_choice(row: dict):
+  """(family, loss): the family whose largest loss against the per-density
+  choices of a table row is smallest, and that loss in ms."""
+  fams = [f for f in FAMILIES if all(f"ms_{f}" in e for e in row.values())]
+  chosen = {d: e[f"ms_{e['choice']}"] for d, e in row.items()}
+  loss = {f: max(e[f"ms_{f}"] - chosen[d] for d, e in row.items()) for f in fams}
+  f = min(fams, key=lambda x: (loss[x], x != row[next(iter(row))]["choice"]))
+  return f, loss[f]
 
 
 def key(connectivity: int, binary: bool, itemsize: int, voxels: int) -> str:
@@ -103,6 +124,9 @@ def tiles(connectivity: int, binary: bool, itemsize: int, voxels: int, is3d: boo
   choices = {v["choice"] for v in row.values()}
   if len(choices) == 1:
     return _FLAG[choices.pop()]
+  fam, loss = unsampled_choice(row)
+  if loss <= sample_ms():
+    return _FLAG[fam]
   d = row.get(density_class(density()))
   return _FLAG[d["choice"]] if d else 0
 
@@ -191,6 +215,19 @@ def measure(reps: int = 5, log=print) -> dict:
   return entries
 
 
+def measure_sample(reps: int = 5) -> float:
+  """Median ms of one density sample of a 192^3 uint32 device volume."""
+  import torch
+  buf = torch.zeros(192 ** 3, dtype=torch.int32, device="cuda")
+  times = []
+  for _ in range(2 + 4 * reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sample_density(buf, 4, buf.numel())
+    times.append((time.perf_counter() - t0) * 1e3)
+  return float(np.median(times[2:]))
+
+
 def main(argv=None) -> int:
   import argparse
   import sys
@@ -203,9 +240,12 @@ def main(argv=None) -> int:
     sys.path.insert(0, here)
   entries = measure(args.reps, log=lambda m: print(m, flush=True))
   import torch
+  smp = measure_sample(args.reps)
+  print(f"[tune] one density sample: {smp:.4f} ms", flush=True)
   doc = {"device": torch.cuda.get_device_name(0), "reps": args.reps,
-         "note": "median ms of one connected_components call (device input, uint32 out) per tile family",
-         "entries": entries}
+         "note": "median ms of one connected_components call (device input, uint32 out) per tile family; "
+                 "sample_ms: median ms of one density sample (sample_density with its host synchronisation)",
+         "sample_ms": round(smp, 4), "entries": entries}
   with open(args.out, "w") as f:
     json.dump(doc, f, indent=1, sort_keys=True)
   print("wrote", args.out)

@@ -516,43 +516,34 @@ __host__ __device__ constexpr int fin_tiles() {
   return fin_ept<G>() >= 16 ? 1 : 16 / fin_ept<G>();
 }
 
-// Root indices of thread t's EPT consecutive J entries of tile `slot` (vector
-// loads), and the tile's label table: entry 0 background, entry 1 + i the
-// label of local root i.
+// Thread t finalizes the 8-voxel columns t, t + WG, ... of the tile (J
+// entries 8c..8c+7 of column c, the local kernel's thread-major order), so
+// for each voxel of a column the workgroup's stores cover whole x-rows.
+// fin_load: the root indices of those entries (vector loads) and the tile's
+// label table: entry 0 background, entry 1 + i the label of local root i.
 template <class G>
 __device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t root0, uint32_t base,
                                 uint32_t (&j)[fin_ept<G>()], uint32_t* lab) {
   constexpr int EPT = fin_ept<G>();
-  static_assert(EPT * WG == G::TV && (EPT == 8 || EPT == 16), "each thread finalizes 8 or 16 consecutive J entries");
+  static_assert(EPT * WG == G::TV && G::PER_THREAD == 8 && EPT % 8 == 0,
+                "each thread finalizes whole 8-voxel columns");
   const int t = threadIdx.x;
   const uint32_t R = p.TR[slot];
   if (R <= 255) {
-    uint32_t ws[EPT / 4];
-    if constexpr (EPT == 8) {
-      const uint2 w = reinterpret_cast<const uint2*>(p.J8 + slot * G::TV)[t];
-      ws[0] = w.x;
-      ws[1] = w.y;
-    } else {
-      const uint4 w = reinterpret_cast<const uint4*>(p.J8 + slot * G::TV)[t];
-      ws[0] = w.x;
-      ws[1] = w.y;
-      ws[2] = w.z;
-      ws[3] = w.w;
-    }
-#pragma unroll
-    for (int k = 0; k < EPT; k++) j[k] = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-  } else {
-    uint32_t ws[EPT / 2];
 #pragma unroll
     for (int h = 0; h < EPT / 8; h++) {
-      const uint4 w = reinterpret_cast<const uint4*>(p.J16 + slot * G::TV)[(EPT / 8) * t + h];
-      ws[4 * h] = w.x;
-      ws[4 * h + 1] = w.y;
-      ws[4 * h + 2] = w.z;
-      ws[4 * h + 3] = w.w;
-    }
+      const uint2 w = reinterpret_cast<const uint2*>(p.J8 + slot * G::TV)[t + WG * h];
 #pragma unroll
-    for (int k = 0; k < EPT; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+      for (int k = 0; k < 8; k++) j[8 * h + k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xFFu;
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < EPT / 8; h++) {
+      const uint4 w = reinterpret_cast<const uint4*>(p.J16 + slot * G::TV)[t + WG * h];
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k = 0; k < 8; k++) j[8 * h + k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+    }
   }
   const uint32_t* __restrict__ LT = p.RL + root0;
   lab[0] = 0u;
@@ -572,7 +563,7 @@ __device__ inline void fin_store(const VolGeo& v, int64_t tile, OUT* __restrict_
 #pragma unroll
   for (int k = 0; k < EPT; k++) {
     int vx, vy, vz;
-    G::entry_voxel(EPT * t + k, vx, vy, vz);
+    G::entry_voxel(8 * (t + WG * (k >> 3)) + (k & 7), vx, vy, vz);
     const int64_t gx = tx * G::TX + vx, gy = ty * G::TY + vy, gz = tz * G::TZ + vz;
     if (gx < v.sx && gy < v.sy && gz < v.sz) out[gx + v.sx * (gy + v.sy * gz)] = (OUT)lab[j[k]];
   }

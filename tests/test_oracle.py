@@ -517,12 +517,28 @@ def test_tile_table_lookup():
   def density():
     calls.append(1)
     return 0.1
-  flag = _tune.tiles(6, False, 4, 400 ** 3, True, density)
-  row = ent[_tune.key(6, False, 4, 400 ** 3)]
-  exp = row[_tune.density_class(0.1)]["choice"] if len({v["choice"] for v in row.values()}) > 1 else \
-    next(iter(row.values()))["choice"]
-  assert flag == {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS, "tall": L.FLAG_TILES_TALL}[exp]
-  assert bool(calls) == (len({v["choice"] for v in row.values()}) > 1)
+  flags = {"rows": L.FLAG_TILES_ROWS, "voxels": L.FLAG_TILES_VOXELS, "tall": L.FLAG_TILES_TALL}
+  for k in ent:
+    conn, mode, isz, cls = k.split("/")
+    voxels = {"s": 64 ** 3, "m": 160 ** 3, "l": 400 ** 3}[cls]
+    calls.clear()
+    flag = _tune.tiles(int(conn), mode == "binary", int(isz), voxels, True, density)
+    row = ent[k]
+    fam, loss = _tune.unsampled_choice(row)
+    sampled = len({v["choice"] for v in row.values()}) > 1 and loss > _tune.sample_ms()
+    if sampled:
+      exp = row[_tune.density_class(0.1)]["choice"]
+    elif len({v["choice"] for v in row.values()}) == 1:
+      exp = next(iter(row.values()))["choice"]
+    else:
+      exp = fam
+    assert flag == flags[exp], k
+    assert bool(calls) == sampled, k
+  # the unsampled family minimises the worst loss against the per-density choices
+  row = {"sparse": {"choice": "voxels", "ms_voxels": 1.0, "ms_tall": 1.01, "ms_rows": 2.0},
+         "dense": {"choice": "tall", "ms_voxels": 1.2, "ms_tall": 1.0, "ms_rows": 2.0}}
+  fam, loss = _tune.unsampled_choice(row)
+  assert fam == "tall" and abs(loss - 0.01) < 1e-9
   assert _tune.tiles(26, False, 4, 100, False, density) == 0  # a single plane: no choice
   g = torch.Generator().manual_seed(0)
   buf = (torch.rand(100000, generator=g) < 0.3).to(torch.int32) * 7
