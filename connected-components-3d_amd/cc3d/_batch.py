@@ -3,11 +3,12 @@
 No counterpart in the reference, which labels one volume per call.  Result i
 of `connected_components_batch(volumes, ...)` is exactly
 `connected_components(volumes[i], ...)`: the same labels, N, output dtype and
-exceptions.  Volumes the batched kernels cover share one launch of every
-kernel (cc3d_amd_batch_* in include/cc3d_amd.h), one batch per input dtype and
-mode (multilabel, binary, continuous): 3D connectivities, or a single plane,
-with or without a periodic boundary.  The rest -- binary 2D-8 and 2D-4,
-continuous 2D-8 and out_file outputs -- go through the single-volume path.
+exceptions.  Volumes share one launch of every kernel (cc3d_amd_batch_* in
+include/cc3d_amd.h), one batch per input dtype and mode (multilabel, binary,
+continuous), whatever their connectivity, and with or without a periodic
+boundary.  The rest -- out_file outputs, and binary 2D-8 images whose
+provisional blocks must be counted for an explicit uint16 output -- go
+through the single-volume path.
 
 `LabelBatch` keeps a planned batch of device-resident volumes with fixed
 output buffers, for repeated labelling; it can capture the whole launch
@@ -32,13 +33,14 @@ def _mode(pl) -> int:
 
 
 def _batchable(pl) -> bool:
+  # FLAG_COUNT_PROVISIONAL (binary 2D-8 into an explicit uint16 output past
+  # 65,535 blocks) has no batch counterpart
   if pl.out_file is not None or (pl.flags & ~L.FLAG_CONTINUOUS):
     return False
   if pl.connectivity in (26, 18, 6):
     return True
   if pl.connectivity in (8, 4):
-    mode = _mode(pl)
-    return pl.sz == 1 and mode != L.BATCH_BINARY and not (mode == L.BATCH_CONTINUOUS and pl.connectivity == 8)
+    return pl.sz == 1
   return False
 
 

@@ -1295,7 +1295,7 @@ struct cc3d_amd_batch {
     int64_t tile0, tiles;  // within tile group tg
   };
   // tile families of a batch (one geometry and set of per-tile arrays each)
-  enum { FAM_VOXELS = 0, FAM_TALL = 1, FAM_ROWS = 2 };
+  enum { FAM_VOXELS = 0, FAM_TALL = 1, FAM_ROWS = 2, FAM_PLANE = 3 };
   struct TileGroup {
     int fam = FAM_VOXELS;
     bool rows() const { return fam == FAM_ROWS; }
@@ -1314,6 +1314,20 @@ struct cc3d_amd_batch {
   std::vector<Group> groups;
   std::vector<TileGroup> tgroups;
   std::vector<Periodic> periodic;
+  // continuous 2D-8 volumes: table entry and offset of their {min, max} block
+  struct MinMax {
+    int k;
+    size_t oMM;
+  };
+  std::vector<MinMax> minmax;
+  // binary 2D-8 volumes with an odd side: table entry, their block bitmask,
+  // prefix and block sums (the reference's stale background labels)
+  struct Stale {
+    int k;
+    int64_t qblocks, qwords, qnblk;
+    size_t oQB, oQP, oQS;
+  };
+  std::vector<Stale> stale;
   bool delta_set = false;
   bool outputs_set = false, capturing = false;
   GlobalParams gp{};  // whole batch: the key space, bitmask, counters
@@ -1382,17 +1396,19 @@ int run_global(cc3d_amd_job* j) {
 }
 
 template <class T>
-int launch_minmax(cc3d_amd_job* j, unsigned long long* mm) {
-  hipStream_t s = j->stream;
-  const T* in = static_cast<const T*>(j->in);
+int launch_minmax_on(const T* in, int64_t V, unsigned long long* mm, hipStream_t s) {
   long long* lastnan = reinterpret_cast<long long*>(mm + 2);
-  const dim3 grid(grid_for(j->g.V, WG, 4096));
+  const dim3 grid(grid_for(V, WG, 4096));
   hipLaunchKernelGGL(k_mm_init, dim3(1), dim3(1), 0, s, mm);
-  if (T(-1) < T(0)) hipLaunchKernelGGL((k_lastnan<T>), grid, dim3(WG), 0, s, in, j->g.V, lastnan);
-  hipLaunchKernelGGL((k_minmax<T>), grid, dim3(WG), 0, s, in, j->g.V, lastnan, mm);
-  hipLaunchKernelGGL((k_minmax_final<T>), dim3(1), dim3(1), 0, s, lastnan, j->g.V, mm);
+  if (T(-1) < T(0)) hipLaunchKernelGGL((k_lastnan<T>), grid, dim3(WG), 0, s, in, V, lastnan);
+  hipLaunchKernelGGL((k_minmax<T>), grid, dim3(WG), 0, s, in, V, lastnan, mm);
+  hipLaunchKernelGGL((k_minmax_final<T>), dim3(1), dim3(1), 0, s, lastnan, V, mm);
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
+}
+template <class T>
+int launch_minmax(cc3d_amd_job* j, unsigned long long* mm) {
+  return launch_minmax_on<T>(static_cast<const T*>(j->in), j->g.V, mm, j->stream);
 }
 
 template <class T>
@@ -1434,6 +1450,60 @@ bool delta_is_zero(int dt, uint64_t bits) {
   if (dt == DT_F32) return from_bits<float>(bits) == 0.0f;
   if (dt == DT_F64) return from_bits<double>(bits) == 0.0;
   return bits == 0;
+}
+
+// Continuous 2D-8 volumes of a batch: each volume's global {min, max}
+// (cc3d/cc3d_continuous.hpp:298-303), before the local kernels read it.
+template <class T>
+hipError_t batch_minmax(const cc3d_amd_batch* b, hipStream_t s) {
+  for (const auto& m : b->minmax) {
+    const BatchVol& v = b->h_tab[m.k];
+    if (launch_minmax_on<T>(static_cast<const T*>(v.in), v.sx * v.sy * v.sz,
+                            reinterpret_cast<unsigned long long*>(b->ws + m.oMM), s) != CC3D_AMD_OK)
+      return hipErrorLaunchFailure;
+  }
+  return hipGetLastError();
+}
+
+// Binary 2D-8 volumes with an odd side: each volume's new-label blocks and
+// their prefix, as the single-volume path's stage 1 forms them (the block
+// count also goes to the volume's provisional counter there).
+template <class T>
+hipError_t batch_q1(const cc3d_amd_batch* b, hipStream_t s) {
+  for (const auto& st : b->stale) {
+    const BatchVol& v = b->h_tab[st.k];
+    uint32_t* qb = reinterpret_cast<uint32_t*>(b->ws + st.oQB);
+    uint32_t* qp = reinterpret_cast<uint32_t*>(b->ws + st.oQP);
+    uint32_t* qs = reinterpret_cast<uint32_t*>(b->ws + st.oQS);
+    hipLaunchKernelGGL(k_zero_words, dim3(grid_for(st.qwords, WG, 8192)), dim3(WG), 0, s, qb, st.qwords);
+    hipLaunchKernelGGL((k_q1_newblocks<T>), dim3(grid_for(st.qblocks, WG, 65536)), dim3(WG), 0, s,
+                       static_cast<const T*>(v.in), v.sx, v.sy, qb, b->gp.ctr + 1 + st.k);
+    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)st.qnblk), dim3(WG), 0, s, qb, st.qwords, qs);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, qs, st.qnblk, (Counters*)nullptr);
+    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)st.qnblk), dim3(WG), 0, s, qb, st.qwords, qs, qp);
+  }
+  return hipGetLastError();
+}
+
+// Their stale background labels, after the volume's output is written.
+template <class T, class OUT>
+void batch_stale_one(const cc3d_amd_batch* b, const cc3d_amd_batch::Stale& st, hipStream_t s) {
+  const BatchVol& v = b->h_tab[st.k];
+  const int64_t V = v.sx * v.sy;
+  const int64_t n = st.qblocks - ((V >> 2) / v.sx) * v.sx;
+  hipLaunchKernelGGL((k_q1_stale<T, OUT>), dim3(grid_for(std::max<int64_t>(n, 1), WG, 4096)), dim3(WG), 0, s,
+                     static_cast<const T*>(v.in), v.sx, v.sy, reinterpret_cast<const uint32_t*>(b->ws + st.oQB),
+                     reinterpret_cast<const uint32_t*>(b->ws + st.oQP), static_cast<OUT*>(v.out));
+}
+template <class T>
+hipError_t batch_stale(const cc3d_amd_batch* b, hipStream_t s) {
+  for (const auto& st : b->stale) {
+    const int od = b->h_tab[st.k].out_dtype;
+    if (od == DT_U16) batch_stale_one<T, uint16_t>(b, st, s);
+    else if (od == DT_U64) batch_stale_one<T, uint64_t>(b, st, s);
+    else batch_stale_one<T, uint32_t>(b, st, s);
+  }
+  return hipGetLastError();
 }
 
 // Wrap-around unions of one periodic volume of a batch: the single-volume
@@ -1923,7 +1993,9 @@ int cc3d_amd_profile_collect(char* names, int name_len, double* total_ms, int* c
 // 2D-4 (their block and first-row numberings) and continuous 2D-8 (its global
 // min/max rule).
 static void batch_tile_dims(int fam, int& TX, int& TY, int& TZ) {
-  if (fam == cc3d_amd_batch::FAM_ROWS) {
+  if (fam == cc3d_amd_batch::FAM_PLANE) {
+    TX = Geo2::TX; TY = Geo2::TY; TZ = 1;
+  } else if (fam == cc3d_amd_batch::FAM_ROWS) {
     TX = GeoR::TX; TY = GeoR::TY; TZ = GeoR::TZ;
   } else if (fam == cc3d_amd_batch::FAM_TALL) {
     TX = Geo3T::TX; TY = Geo3T::TY; TZ = Geo3T::TZ;
@@ -1940,7 +2012,7 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
   tab.resize(n);
   b.perm.resize(n);
   b.groups.clear();
-  std::vector<int> nbs(n), nbps(n);
+  std::vector<int> nbs(n), nbps(n), qks(n);
   std::vector<int> famv(n);
   for (int i = 0; i < n; i++) {  // checked in the caller's order
     const cc3d_amd_volume& v = vols[i];
@@ -1952,13 +2024,12 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
          ~(int64_t)((CC3D_AMD_TILES_ROWS | CC3D_AMD_TILES_VOXELS | CC3D_AMD_TILES_TALL) << 8 |
                     CC3D_AMD_VOLUME_PERIODIC | 0xFF)) != 0)
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
+    qks[i] = Q_NONE;
     if (nb == 8 || nb == 4) {
       if (v.sz != 1) return fail(CC3D_AMD_ERR_RUNTIME, "sz must be 1 for 2D connectivities.");
-      if (mode == CC3D_AMD_BATCH_BINARY)
-        return fail(CC3D_AMD_ERR_INVALID, "binary 2D-8 and 2D-4 volumes are not supported in a batch");
-      if (mode == CC3D_AMD_BATCH_CONTINUOUS && nb == 8)
-        return fail(CC3D_AMD_ERR_INVALID, "continuous 2D-8 volumes are not supported in a batch");
-      nb = nb == 8 ? 26 : 6;  // a single plane: the same links
+      if (mode == CC3D_AMD_BATCH_BINARY) qks[i] = nb == 8 ? Q_BLOCK8 : Q_BIN4;
+      else if (mode == CC3D_AMD_BATCH_CONTINUOUS && nb == 8) qks[i] = Q_CONT8;
+      else nb = nb == 8 ? 26 : 6;  // a single plane: the same links on the 3D tiles
     } else if (nb != 26 && nb != 18 && nb != 6) {
       return fail(CC3D_AMD_ERR_RUNTIME, "Only 4 and 8 2D and 6, 18, and 26 3D connectivities are supported.");
     }
@@ -1974,7 +2045,8 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
     // voxels, then tall; by default rows at 6-connectivity, else voxels
     const int pref = (int)((v.connectivity >> 8) & 0xFF);
     const bool cont = mode == CC3D_AMD_BATCH_CONTINUOUS;
-    famv[i] = (!cont && (pref & CC3D_AMD_TILES_ROWS))  ? cc3d_amd_batch::FAM_ROWS
+    famv[i] = qks[i] != Q_NONE                           ? cc3d_amd_batch::FAM_PLANE
+              : (!cont && (pref & CC3D_AMD_TILES_ROWS))  ? cc3d_amd_batch::FAM_ROWS
               : (pref & CC3D_AMD_TILES_VOXELS)          ? cc3d_amd_batch::FAM_VOXELS
               : (pref & CC3D_AMD_TILES_TALL)            ? cc3d_amd_batch::FAM_TALL
               : (!cont && nb == 6)                      ? cc3d_amd_batch::FAM_ROWS
@@ -1988,6 +2060,8 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
   uint64_t key = 0;
   b.tgroups.clear();
   b.periodic.clear();
+  b.minmax.clear();
+  b.stale.clear();
   for (int k = 0; k < n; k++) {
     const int i = b.perm[k], nb = nbs[i], fam = famv[i];
     if (b.tgroups.empty() || b.tgroups.back().fam != fam) {
@@ -2013,17 +2087,30 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
     t.ntiles = t.ntx * t.nty * t.ntz;
     t.tile0 = tg.tiles;
     const uint64_t V = (uint64_t)v.sx * (uint64_t)v.sy * (uint64_t)v.sz;
+    // binary 2D-8 keys are 2x2 block keys: 4 per block (plan's K)
+    const uint64_t K = qks[i] == Q_BLOCK8 ? 4ull * (uint64_t)t.osx * (uint64_t)((v.sy + 1) >> 1) : V;
     t.key0 = (uint32_t)key;
-    t.nkeys = (uint32_t)std::min<uint64_t>(V, 0xFFFFFFFFull);
+    t.nkeys = (uint32_t)std::min<uint64_t>(K, 0xFFFFFFFFull);
     t.nb = nb;
     t.out_dtype = DT_U32;
+    t.qk = qks[i];
+    t.minmax = nullptr;
+    if (qks[i] == Q_CONT8) b.minmax.push_back({k, 0});
+    if (qks[i] == Q_BLOCK8 && ((v.sx & 1) || (v.sy & 1))) {
+      cc3d_amd_batch::Stale st{};
+      st.k = k;
+      st.qblocks = t.osx * ((v.sy + 1) >> 1);
+      st.qwords = (st.qblocks + 31) / 32;
+      st.qnblk = (st.qwords + SCAN_WPB - 1) / SCAN_WPB;
+      b.stale.push_back(st);
+    }
     if (b.groups.empty() || b.groups.back().nb != nb || b.groups.back().tg != (int)b.tgroups.size() - 1)
       b.groups.push_back({nb, (int)b.tgroups.size() - 1, tg.tiles, 0});
     b.groups.back().tiles += t.ntiles;
     if (nbps[i]) b.periodic.push_back({k, nbps[i], (int)b.tgroups.size() - 1});
     tg.tiles += t.ntiles;
     tg.v1 = k + 1;
-    key += (V + 31) & ~(uint64_t)31;
+    key += (K + 31) & ~(uint64_t)31;
     if (key >= 0xFFFFFFFFull) return fail(CC3D_AMD_ERR_INVALID, "a batch must hold fewer than 2^32 - 1 voxels");
   }
   // first tile of each volume within its tile group; tile group g's entries
@@ -2057,7 +2144,10 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
     int TXg, TYg, TZg;
     batch_tile_dims(tg.fam, TXg, TYg, TZg);
     const size_t TVg = (size_t)TXg * TYg * TZg;
-    const size_t HALOg = tg.rows() ? GeoR::HALO : tg.fam == cc3d_amd_batch::FAM_TALL ? Geo3T::HALO : Geo3::HALO;
+    const size_t HALOg = tg.rows()                              ? GeoR::HALO
+                         : tg.fam == cc3d_amd_batch::FAM_TALL  ? Geo3T::HALO
+                         : tg.fam == cc3d_amd_batch::FAM_PLANE ? Geo2::HALO
+                                                               : Geo3::HALO;
     const size_t slots = (size_t)tg.tiles * TVg;
     tg.oT0 = take((size_t)(tg.v1 - tg.v0 + 1) * sizeof(int64_t));
     tg.oVIX = take((size_t)tg.tiles * 4);
@@ -2067,6 +2157,12 @@ static int batch_layout(const cc3d_amd_volume* vols, int n, int mode, std::vecto
     tg.oJ8 = take(slots);
     tg.oJ16 = take(slots * 2);
     tg.oREQ = take((size_t)tg.tiles * HALOg * 4);
+  }
+  for (auto& m : b.minmax) m.oMM = take(64);  // {min, max} and the last NaN index
+  for (auto& st : b.stale) {
+    st.oQB = take((size_t)st.qwords * 4);
+    st.oQP = take((size_t)st.qwords * 4);
+    st.oQS = take((size_t)(st.qnblk + 1) * 4);
   }
   b.total = o;
   return CC3D_AMD_OK;
@@ -2123,6 +2219,7 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     cc3d_amd_batch_free(b);
     return fail(CC3D_AMD_ERR_NOMEM, "could not allocate pinned batch tables");
   }
+  for (const auto& m : b->minmax) tab[m.k].minmax = reinterpret_cast<const uint64_t*>(b->ws + m.oMM);
   std::memcpy(b->h_tab, tab.data(), n * sizeof(BatchVol));
   std::memcpy(b->h_t0, t0.data(), t0.size() * sizeof(int64_t));
   char* ws = b->ws;
@@ -2238,12 +2335,30 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   }
   HIPCHECK(hipGetLastError());
   hipError_t e = hipSuccess;
+#define CC3D_BDT(FN)                                 \
+  switch (b->dt) {                                   \
+    case DT_U8: e = FN<uint8_t>(b, s); break;        \
+    case DT_U16: e = FN<uint16_t>(b, s); break;      \
+    case DT_U32: e = FN<uint32_t>(b, s); break;      \
+    case DT_U64: e = FN<uint64_t>(b, s); break;      \
+    case DT_F32: e = FN<float>(b, s); break;         \
+    default: e = FN<double>(b, s); break;            \
+  }
+  if (!b->minmax.empty()) {
+    ProfScope ps_("k_minmax", s);
+    CC3D_BDT(batch_minmax);
+    if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_minmax: ") + hipGetErrorString(e));
+  }
   {
     ProfScope ps_("k_local", s);
     for (const auto& g : b->groups) {
       const auto& tg = b->tgroups[g.tg];
       const bool tall = tg.fam == cc3d_amd_batch::FAM_TALL;
-      if (tg.rows() && b->mode == M_BINARY)
+      if (tg.fam == cc3d_amd_batch::FAM_PLANE && b->mode == M_BINARY)
+        e = launch_local_batch_plane_binary(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+      else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
+        e = launch_local_batch_plane_cont(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
+      else if (tg.rows() && b->mode == M_BINARY)
         e = launch_rows_batch_binary(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
       else if (tg.rows())
         e = launch_rows_batch_multi(b->dt, g.nb, tg.lp, tg.gp.vols, tg.gp.vix, g.tile0, g.tiles, s);
@@ -2271,6 +2386,8 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
       if (tg.rows()) hipLaunchKernelGGL((k_requests<GeoR, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else if (tg.fam == cc3d_amd_batch::FAM_TALL)
         hipLaunchKernelGGL((k_requests<Geo3T, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
+        hipLaunchKernelGGL((k_requests<Geo2, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
     }
   }
@@ -2279,9 +2396,10 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
     ProfScope ps_("k_periodic", s);
     for (const auto& pv : b->periodic) {
       const int fam = b->tgroups[pv.tg].fam;
-      e = fam == cc3d_amd_batch::FAM_ROWS   ? batch_periodic<GeoR>(b, pv, s)
-          : fam == cc3d_amd_batch::FAM_TALL ? batch_periodic<Geo3T>(b, pv, s)
-                                            : batch_periodic<Geo3>(b, pv, s);
+      e = fam == cc3d_amd_batch::FAM_ROWS    ? batch_periodic<GeoR>(b, pv, s)
+          : fam == cc3d_amd_batch::FAM_TALL  ? batch_periodic<Geo3T>(b, pv, s)
+          : fam == cc3d_amd_batch::FAM_PLANE ? batch_periodic<Geo2>(b, pv, s)
+                                             : batch_periodic<Geo3>(b, pv, s);
       if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_periodic: ") + hipGetErrorString(e));
     }
   }
@@ -2292,6 +2410,8 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
       if (tg.rows()) hipLaunchKernelGGL((k_roots<GeoR>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else if (tg.fam == cc3d_amd_batch::FAM_TALL)
         hipLaunchKernelGGL((k_roots<Geo3T>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
+        hipLaunchKernelGGL((k_roots<Geo2>), dim3(blocks), dim3(WG), 0, s, tg.gp);
       else hipLaunchKernelGGL((k_roots<Geo3>), dim3(blocks), dim3(WG), 0, s, tg.gp);
     }
   }
@@ -2307,6 +2427,11 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
                        p.ctr + 1);
   }
   HIPCHECK(hipGetLastError());
+  if (!b->stale.empty()) {
+    CC3D_BDT(batch_q1);
+    if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_q1: ") + hipGetErrorString(e));
+  }
+#undef CC3D_BDT
   return CC3D_AMD_OK;
 }
 
@@ -2336,11 +2461,27 @@ int cc3d_amd_batch_finish(cc3d_amd_batch* b) {
     else if (tg.fam == cc3d_amd_batch::FAM_TALL)
       hipLaunchKernelGGL((k_finalize_batch<Geo3T>), dim3((unsigned)((tg.tiles + fin_tiles<Geo3T>() - 1) / fin_tiles<Geo3T>())),
                          dim3(WG), 0, b->stream, tg.gp);
+    else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
+      hipLaunchKernelGGL((k_finalize_batch<Geo2>), dim3((unsigned)((tg.tiles + fin_tiles<Geo2>() - 1) / fin_tiles<Geo2>())),
+                         dim3(WG), 0, b->stream, tg.gp);
     else
       hipLaunchKernelGGL((k_finalize_batch<Geo3>), dim3((unsigned)((tg.tiles + fin_tiles<Geo3>() - 1) / fin_tiles<Geo3>())),
                          dim3(WG), 0, b->stream, tg.gp);
   }
   HIPCHECK(hipGetLastError());
+  if (!b->stale.empty()) {
+    hipError_t e;
+    hipStream_t s = b->stream;
+    switch (b->dt) {
+      case DT_U8: e = batch_stale<uint8_t>(b, s); break;
+      case DT_U16: e = batch_stale<uint16_t>(b, s); break;
+      case DT_U32: e = batch_stale<uint32_t>(b, s); break;
+      case DT_U64: e = batch_stale<uint64_t>(b, s); break;
+      case DT_F32: e = batch_stale<float>(b, s); break;
+      default: e = batch_stale<double>(b, s); break;
+    }
+    if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_q1_stale: ") + hipGetErrorString(e));
+  }
   return CC3D_AMD_OK;
 }
 

@@ -53,6 +53,8 @@ struct BatchVol {
   uint32_t key0;      // first key of this volume in the batch's key space (multiple of 32)
   uint32_t nkeys;
   int nb, out_dtype;
+  int qk;                   // numbering / link variant of a single plane on the 2D tiles (Q_*)
+  const uint64_t* minmax;   // continuous 2D-8: the volume's global {min, max} bit patterns
 };
 
 // Tile geometry.  A tile is TX x TY x TZ voxels; the local kernel's

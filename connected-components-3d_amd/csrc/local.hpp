@@ -1222,7 +1222,7 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>()
 // workgroup reads its volume's table entry from vix.  p carries the shared per-tile
 // arrays; the geometry fields come from the table.  Queue capacity and
 // occupancy are those of the single-volume kernel of the same connectivity.
-template <class T, int MODE, int NB, class G>
+template <class T, int MODE, int NB, class G, int QK = Q_NONE>
 __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>())) void k_local_batch(
   LocalParams p, const BatchVol* vols, const uint32_t* vix, int64_t gtile0, int64_t gtiles) {
   __shared__ LocalLds<T, G, full_queue<MODE, NB>()> L;
@@ -1238,7 +1238,8 @@ __global__ __launch_bounds__(G::NT, (local_waves<T, G, full_queue<MODE, NB>()>()
   q.nty = v.nty;
   q.ntz = v.ntz;
   q.ntiles = v.ntiles;
-  local_tile<T, MODE, NB, Q_NONE, G, full_queue<MODE, NB>()>(q, slot - v.tile0, slot, v.key0, L);
+  q.minmax = v.minmax;
+  local_tile<T, MODE, NB, QK, G, full_queue<MODE, NB>()>(q, slot - v.tile0, slot, v.key0, L);
 }
 
 // Launchers (one translation unit per mode keeps compile times short).
@@ -1254,6 +1255,41 @@ hipError_t launch_local_batch_binary(int dt, int nb, bool tall, const LocalParam
                                      const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
 hipError_t launch_local_batch_cont(int dt, int nb, bool tall, const LocalParams& p, const BatchVol* vols,
                                    const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
+// Batch volumes on the 2D tiles (single planes whose numbering or links need
+// the 2D kernel): binary 2D-4 (Q_BIN4) and 2D-8 (Q_BLOCK8), continuous 2D-8
+// (Q_CONT8).
+hipError_t launch_local_batch_plane_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols,
+                                           const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
+hipError_t launch_local_batch_plane_cont(int dt, int nb, const LocalParams& p, const BatchVol* vols,
+                                         const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s);
+
+template <int MODE>
+hipError_t launch_local_batch_plane_mode(int dt, int nb, const LocalParams& p, const BatchVol* vols,
+                                         const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s) {
+  const dim3 grid((unsigned)gtiles), block(Geo2::NT);
+  constexpr int Q8 = MODE == M_BINARY ? Q_BLOCK8 : MODE == M_CONT ? Q_CONT8 : Q_NONE;
+  constexpr int Q4 = MODE == M_BINARY ? Q_BIN4 : Q_NONE;
+#define CC3D_P(T)                                                                                                 \
+  do {                                                                                                            \
+    if (nb == 8 && Q8 != Q_NONE)                                                                                  \
+      hipLaunchKernelGGL((k_local_batch<T, MODE, 8, Geo2, Q8>), grid, block, 0, s, p, vols, vix, gtile0, gtiles); \
+    else if (nb == 4 && Q4 != Q_NONE)                                                                             \
+      hipLaunchKernelGGL((k_local_batch<T, MODE, 4, Geo2, Q4>), grid, block, 0, s, p, vols, vix, gtile0, gtiles); \
+    else                                                                                                          \
+      return hipErrorInvalidValue;                                                                                \
+  } while (0)
+  switch (dt) {
+    case DT_U8: CC3D_P(uint8_t); break;
+    case DT_U16: CC3D_P(uint16_t); break;
+    case DT_U32: CC3D_P(uint32_t); break;
+    case DT_U64: CC3D_P(uint64_t); break;
+    case DT_F32: CC3D_P(float); break;
+    case DT_F64: CC3D_P(double); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef CC3D_P
+  return hipGetLastError();
+}
 
 template <int MODE, class G>
 hipError_t launch_local_batch_geo(int dt, int nb, const LocalParams& p, const BatchVol* vols, const uint32_t* vix,

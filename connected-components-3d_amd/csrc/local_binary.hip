@@ -9,4 +9,8 @@ hipError_t launch_local_batch_binary(int dt, int nb, bool tall, const LocalParam
                                    const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s) {
   return launch_local_batch_mode<M_BINARY>(dt, nb, tall, p, vols, vix, gtile0, gtiles, s);
 }
+hipError_t launch_local_batch_plane_binary(int dt, int nb, const LocalParams& p, const BatchVol* vols,
+                                         const uint32_t* vix, int64_t gtile0, int64_t gtiles, hipStream_t s) {
+  return launch_local_batch_plane_mode<M_BINARY>(dt, nb, p, vols, vix, gtile0, gtiles, s);
+}
 }  // namespace cc3d_amd

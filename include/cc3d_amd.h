@@ -153,10 +153,9 @@ uint64_t cc3d_amd_union_find_length(int64_t max_labels, int64_t voxels, int64_t 
  * volumes fill the GPU and a batch costs one host synchronisation.  One mode
  * per batch: multilabel, binary or continuous (one delta for all volumes).
  * Per volume: connectivity 26, 18 or 6, or 8 and 4 on a single plane, and
- * optionally a periodic boundary (not continuous).  Not covered: binary
- * 2D-8 and 2D-4 volumes and continuous 2D-8 volumes (connectivity 8, or 8
- * and 4 for binary, given for a plane).  The sum of the volumes' voxels must
- * stay below 2^32 - 32 * n. */
+ * optionally a periodic boundary (not continuous).  The sum of the volumes'
+ * keys (voxels; 4 per 2x2 block for binary 2D-8) must stay below
+ * 2^32 - 32 * n. */
 typedef struct {
   const void* in;          /* device pointer, dense, x fastest */
   int64_t sx, sy, sz;

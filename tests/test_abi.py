@@ -98,7 +98,11 @@ def test_batch_layout_with_every_tile_family():
   assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_CONTINUOUS) == 0  # periodic + continuous
   vt[2].connectivity = 8
   vt[2].sz = 1
-  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_MULTILABEL) > 0
-  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_BINARY) == 0  # binary 2D-8
-  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_CONTINUOUS) == 0  # continuous 2D-8
+  multi = lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_MULTILABEL)
+  binary = lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_BINARY)  # the plane on the 2D tiles, block keys
+  cont = lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_CONTINUOUS)  # the plane on the 2D tiles, its min/max
+  assert multi > 0 and binary > 0 and cont > 0
+  vt[2].sz = 2  # 2D connectivities need a single plane
+  assert lib.cc3d_amd_batch_workspace_bytes(vt, n, L.BATCH_BINARY) == 0
+  vt[2].sz = 1
   assert lib.cc3d_amd_batch_workspace_bytes(vt, n, 3) == 0  # no such mode

@@ -212,7 +212,7 @@ def _job_stats(v, sx, sy, sz, conn, code, mode, delta_b, periodic):
   return st
 
 
-@pytest.mark.parametrize("mode,periodic", [(0, True), (1, True), (1, False), (2, False)])
+@pytest.mark.parametrize("mode,periodic", [(0, True), (1, True), (1, False), (2, False), (0, False)])
 def test_batch_stats_match_single_entry(gpu, mode, periodic):
   """The batch's stage-1 counters (N, provisional labels, epl, foreground
   rows) equal the single-volume entry's for every volume, through the C ABI."""
@@ -221,13 +221,10 @@ def test_batch_stats_match_single_entry(gpu, mode, periodic):
   from cc3d import _lib as L
   rng = np.random.default_rng(20 + mode)
   # device volumes stored x fastest: torch shape (sz, sy, sx) of a C-order array
-  shapes = [((30, 41, 17), 6), ((64, 33, 1), 4), ((50, 60, 1), 8), ((20, 20, 20), 6), ((97, 1, 1), 6)]
+  shapes = [((30, 41, 17), 6), ((64, 33, 1), 4), ((50, 60, 1), 8), ((20, 20, 20), 6), ((97, 1, 1), 6),
+            ((51, 37, 1), 8), ((70, 1, 1), 8)]
   if not periodic:
     shapes += [((40, 30, 20), 26), ((40, 30, 20), 18), ((77, 45, 1), 26)]
-  if mode == L.BATCH_BINARY:
-    shapes = [(s, c) for s, c in shapes if not (s[2] == 1 and c in (4, 8))]
-  if mode == L.BATCH_CONTINUOUS:
-    shapes = [(s, c) for s, c in shapes if not (s[2] == 1 and c == 8)]
   dt = np.float32 if mode == L.BATCH_CONTINUOUS else np.uint8
   code = L.DT_F32 if dt == np.float32 else L.DT_U8
   delta_b = np.array([3.0], dtype=np.float32).tobytes() if mode == L.BATCH_CONTINUOUS else b""
@@ -300,3 +297,30 @@ def test_label_batch_continuous_graph_replays(gpu):
       assert N == eager[k][1] and torch.equal(o, eager[k][0]), f"replay {rep} volume {k}"
     del junk
   lb.close()
+
+
+@pytest.mark.parametrize("binary,conn", [(True, 8), (True, 4), (False, 8)])
+def test_batch_2d_images_match_single(gpu, binary, conn):
+  """2D images with 2D connectivities: binary 2D-8 (block numbering and, for
+  odd sides, the reference's stale background labels), binary 2D-4, and
+  continuous 2D-8 (global min/max rule) batched on the 2D tiles."""
+  import cc3d
+  import synth
+  rng = np.random.default_rng(40 + conn + binary)
+  vols = []
+  for k, shape in enumerate([(130, 77), (64, 64), (33, 201), (1, 57), (90, 1), (255, 129), (8, 8)]):
+    if binary:
+      v = synth.bernoulli(shape + (1,), 0.45, seed=k)[:, :, 0]
+    else:
+      v = _tones(rng, shape, np.float32)
+    vols.append(np.asfortranarray(v) if k % 2 else v)
+  kw = dict(binary_image=True) if binary else dict(delta=6.5)
+  got = cc3d.connected_components_batch(vols, connectivity=conn, return_N=True, **kw)
+  for i, v in enumerate(vols):
+    exp = cc3d.connected_components(v, connectivity=conn, return_N=True, **kw)
+    assert _same(got[i], exp), f"volume {i} shape={v.shape}"
+  if binary:  # periodic planes on the 2D tiles as well
+    got = cc3d.connected_components_batch(vols, connectivity=conn, return_N=True, periodic_boundary=True, **kw)
+    for i, v in enumerate(vols):
+      exp = cc3d.connected_components(v, connectivity=conn, return_N=True, periodic_boundary=True, **kw)
+      assert _same(got[i], exp), f"periodic volume {i} shape={v.shape}"
