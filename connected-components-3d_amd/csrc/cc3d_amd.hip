@@ -3,16 +3,17 @@
 // Pipeline for one volume (stream-ordered, no host round trip until stats):
 //   k_init       reset counters, clear the root bitmask
 //   [k_minmax]   continuous 2D-8 only: global min/max (cc3d/cc3d_continuous.hpp:298-303)
-//   k_local      per tile: LDS union-find, local roots, J tables, requests
+//   k_local      per tile: LDS union-find, local roots, J tables, requests; every
+//                local root sets the bit of its key in the root bitmask
 //   k_requests   per tile: cross-tile unions on the global forest GP over local
 //                roots (entry tile_root0 + r per local root r), each tree rooted at
-//                its smallest key
+//                its smallest key; a link clears the bit of the root it buries,
+//                so the bits left mark the global roots (kunion)
 //   [k_periodic] wrap-around unions (cc3d/cc3d.hpp:1048-1073, :1265-1277, :1377-1418)
-//   k_roots      per local root: find its global root, store the root's key in RL;
-//                global roots set a bit at their key
 //   k_scan_*     exclusive popcount prefix over the bitmask -> N
 //   --- stage 2 ---
-//   k_finalize   per local root: final label = 1 + rank of its global root's key;
+//   k_finalize   per local root: its global root's key (a find on the settled
+//                forest), final label = 1 + rank of that key;
 //                per voxel: label of its local root, written in the output dtype
 //   [k_q1_*]     binary 2D-8 only: provisional count and stale background labels
 //
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
       G::halo_coords(rq & 0xFFFF, bx, by, bz);
       const int64_t gx = x0 + bx - 1, gy = y0 + by - 1, gz = G::IS3D ? z0 + bz - 1 : 0;
       const uint32_t b = lroot_index<G>(p, v, gx, gy, gz);
-      if (b != 0xFFFFFFFFu) kunion(p.GP, p.RL, a, b);
+      if (b != 0xFFFFFFFFu) kunion(p.GP, p.RL, p.bits, a, b);
     }
   }
 }
@@ -356,32 +357,7 @@ __global__ __launch_bounds__(WG) void k_periodic(GlobalParams p, const T* in, in
     if (!link) continue;
     const VolGeo v = vol_geo(p, 0);
     const uint32_t ka = lroot_index<G>(p, v, ax, ay, az), kb = lroot_index<G>(p, v, bx, by, bz);
-    if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) kunion(p.GP, p.RL, ka, kb);
-  }
-}
-
-// Global roots: every local root finds its global root (the component's
-// smallest key); roots mark their key, and RL takes the root's key.  A root's
-// own entry is never rewritten, so the keys read stay valid.
-template <class G>
-__global__ __launch_bounds__(WG) void k_roots(GlobalParams p) {
-  const int sub = threadIdx.x % TILE_LANES;
-  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
-  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
-  for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
-    const uint32_t R = p.TR[slot];
-    if (R == 0) continue;
-    const uint32_t root0 = slot_root0<G>(vol_geo(p, slot), slot);
-    for (uint32_t j = sub; j < R; j += TILE_LANES) {
-      const uint32_t c = root0 + j;
-      const uint32_t r = kfind_settled(p.GP, c);
-      if (r == c) {
-        const uint32_t key = p.RL[c];
-        atomicOr(p.bits + (key >> 5), 1u << (key & 31));
-      } else {
-        p.RL[c] = p.RL[r];  // from here on RL holds each local root's global root key
-      }
-    }
+    if (ka != 0xFFFFFFFFu && kb != 0xFFFFFFFFu) kunion(p.GP, p.RL, p.bits, ka, kb);
   }
 }
 
@@ -493,11 +469,10 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
 // 16-byte load, issued together with a coalesced copy of the tile's label
 // table into LDS, so the per-voxel lookups hit LDS instead of scattered
 // global reads.
-// Final label of a local root whose global root is r (k_roots stored r in
-// RL): 1 + the rank of r among the global roots, which k_roots marked in the
-// bitmask (prefix: the bitmask's exclusive popcount prefix per word).  base:
-// the prefix at the volume's first key word (a batch numbers each volume
-// from 1).
+// Final label of a local root whose global root has key r: 1 + the rank of r
+// among the global roots' keys, the bits left set in the bitmask (prefix: the
+// bitmask's exclusive popcount prefix per word).  base: the prefix at the
+// volume's first key word (a batch numbers each volume from 1).
 __device__ inline uint32_t root_label(const GlobalParams& p, uint32_t r, uint32_t base) {
   const uint32_t wd = r >> 5;
   return p.prefix[wd] - base + __popc(p.bits[wd] & ((1u << (r & 31)) - 1u)) + 1u;
@@ -545,9 +520,8 @@ __device__ inline void fin_load(const GlobalParams& p, int64_t slot, uint32_t ro
       for (int k = 0; k < 8; k++) j[8 * h + k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
     }
   }
-  const uint32_t* __restrict__ LT = p.RL + root0;
   lab[0] = 0u;
-  for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
+  for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, kroot_key(p.GP, p.RL, root0 + i), base);
 }
 
 // Thread t's EPT output voxels of tile `tile` of volume v (after a barrier
@@ -658,9 +632,9 @@ __device__ inline void rfin_tile(const GlobalParams& p, const VolGeo& v, int64_t
 #pragma unroll
     for (int k = 0; k < RFIN_PT; k++) j[k] = (ws[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
   }
-  const uint32_t* __restrict__ LT = p.RL + slot_root0<GeoR>(v, slot);
+  const uint32_t root0 = slot_root0<GeoR>(v, slot);
   if (t == 0) lab[0] = 0u;
-  for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, LT[i], base);
+  for (uint32_t i = t; i < R; i += WG) lab[i + 1] = root_label(p, kroot_key(p.GP, p.RL, root0 + i), base);
   __syncthreads();
   const int e0 = RFIN_PT * t;
   const int lx = e0 & 63, ly = (e0 >> 6) & (GeoR::TY - 1), lz = e0 >> 9;
@@ -1381,11 +1355,6 @@ int run_global(cc3d_amd_job* j) {
     HIPCHECK(hipGetLastError());
   }
   {
-    ProfScope ps_("k_roots", s);
-    hipLaunchKernelGGL((k_roots<G>), dim3(wave_blocks), dim3(WG), 0, s, p);
-  }
-  HIPCHECK(hipGetLastError());
-  {
   ProfScope ps_("k_scan", s);
   hipLaunchKernelGGL(k_scan_a, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
   hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)j->g.nblk, p.ctr);
@@ -1684,6 +1653,7 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
   lp.minmax = reinterpret_cast<const uint64_t*>(mm);
   lp.GP = gp.GP;
   lp.RL = gp.RL;
+  lp.bits = gp.bits;
   lp.TR = reinterpret_cast<uint32_t*>(ws + g.oTR);
   lp.J8 = reinterpret_cast<uint8_t*>(ws + g.oJ8);
   lp.J16 = reinterpret_cast<uint16_t*>(ws + g.oJ16);
@@ -2272,6 +2242,7 @@ int cc3d_amd_batch_create(cc3d_amd_batch** out_batch, const cc3d_amd_volume* vol
     lp.minmax = nullptr;
     lp.GP = gp.GP;
     lp.RL = q.RL;
+    lp.bits = gp.bits;
     lp.TR = reinterpret_cast<uint32_t*>(ws + tg.oTR);
     lp.J8 = reinterpret_cast<uint8_t*>(ws + tg.oJ8);
     lp.J16 = reinterpret_cast<uint16_t*>(ws + tg.oJ16);
@@ -2403,19 +2374,6 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
       if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_periodic: ") + hipGetErrorString(e));
     }
   }
-  {
-    ProfScope ps_("k_roots", s);
-    for (const auto& tg : b->tgroups) {
-      const unsigned blocks = grid_for(tg.tiles, WG / TILE_LANES, 65536);
-      if (tg.rows()) hipLaunchKernelGGL((k_roots<GeoR>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else if (tg.fam == cc3d_amd_batch::FAM_TALL)
-        hipLaunchKernelGGL((k_roots<Geo3T>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
-        hipLaunchKernelGGL((k_roots<Geo2>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else hipLaunchKernelGGL((k_roots<Geo3>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-    }
-  }
-
   {
     ProfScope ps_("k_scan", s);
     hipLaunchKernelGGL(k_scan_a, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);

@@ -385,32 +385,44 @@ __device__ inline uint32_t kfind(uint32_t* GP, uint32_t a) {
   if (depth >= 4) __hip_atomic_store(GP + n2, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return r;
 }
-__device__ inline void kunion(uint32_t* GP, const uint32_t* RL, uint32_t a, uint32_t b) {
+// Root marks: the local kernel sets the bit of every local root's key, and a
+// link clears the key of the root it buries unless the root it joins has the
+// same key.  Once the unions are done the bits left are the global roots'
+// keys: an entry whose key is not its component's minimum ends below a root
+// with a smaller key, so some link on its path took its key to a smaller one
+// and cleared it; a root with the minimum key is only ever linked under an
+// entry with that same key (entries sharing a key are one component), which
+// clears nothing.
+__device__ inline void kunion(uint32_t* GP, const uint32_t* RL, uint32_t* bits, uint32_t a, uint32_t b) {
   while (true) {
     a = kfind(GP, a);
     b = kfind(GP, b);
     if (a == b) return;
-    const uint64_t ka = ((uint64_t)RL[a] << 32) | a, kb = ((uint64_t)RL[b] << 32) | b;
+    uint64_t ka = ((uint64_t)RL[a] << 32) | a, kb = ((uint64_t)RL[b] << 32) | b;
     if (ka < kb) {
       const uint32_t t = a;
       a = b;
       b = t;
+      const uint64_t tk = ka;
+      ka = kb;
+      kb = tk;
     }
-    if (atomicCAS(GP + a, a, b) == a) return;  // a was still a root: linked under b
+    if (atomicCAS(GP + a, a, b) == a) {  // a was still a root: linked under b
+      const uint32_t key = (uint32_t)(ka >> 32);
+      if (key != (uint32_t)(kb >> 32)) atomicAnd(bits + (key >> 5), ~(1u << (key & 31)));
+      return;
+    }
   }
 }
-// Find once every union is made: only compression remains, so any ancestor
-// stored into a node still leads to its root.
-__device__ inline uint32_t kfind_settled(uint32_t* GP, uint32_t a) {
-  uint32_t r = a;
-  for (uint32_t p = GP[r]; p != r; p = GP[r]) r = p;
-  while (a != r) {
-    const uint32_t p = GP[a];
-    if (p == r) break;
-    GP[a] = r;
-    a = p;
-  }
-  return r;
+// Key of the global root of entry a, once every union is made (plain loads:
+// the forest no longer changes).  Most entries are roots, so the entry's own
+// key is loaded together with its parent.
+__device__ inline uint32_t kroot_key(const uint32_t* GP, const uint32_t* RL, uint32_t a) {
+  uint32_t p = GP[a];
+  const uint32_t k = RL[a];
+  if (p == a) return k;
+  for (a = p, p = GP[a]; p != a; p = GP[a]) a = p;
+  return RL[a];
 }
 
 // Global union-find over keys.  Loads bypass the non-coherent L1 so retries see
@@ -419,9 +431,9 @@ __device__ inline uint32_t gload(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Path halving with atomicMin to the grandparent: parents only ever decrease,
-// so a node that k_roots has pointed straight at its root (the minimum of its
-// set) can never be moved back to an intermediate ancestor by a concurrent
-// find on a longer path.
+// so a node already pointed straight at its root (the minimum of its set) can
+// never be moved back to an intermediate ancestor by a concurrent find on a
+// longer path.
 __device__ inline uint32_t gfind(uint32_t* GP, uint32_t a) {
   uint32_t p = gload(GP + a);
   while (p != a) {

@@ -44,6 +44,8 @@
 //                    8t..8t+7): 0 (background) or 1 + index of its local root;
 //                    8-bit when R <= 255, else 16-bit
 //   GP[root0 + r]    initialised to itself for every local root
+//   bits             bit (key) set for every local root's key (kunion clears
+//                    those of the roots it buries)
 //   REQ/TQ           cross-tile union requests: (root index r, halo index)
 //                    for each halo "piece" (halo voxels connected through links
 //                    that stay inside one neighbouring tile) touching a local
@@ -66,6 +68,7 @@ struct LocalParams {
   const uint64_t* minmax;  // Q_CONT8: global {min, max} bit patterns
   uint32_t* GP;
   uint32_t* RL;
+  uint32_t* bits;  // root bitmask over keys, cleared beforehand
   uint32_t* TR;
   uint8_t* J8;
   uint16_t* J16;
@@ -1118,6 +1121,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
         key = kbox0 + (uint32_t)sx * (uint32_t)sy + (uint32_t)bx + (uint32_t)sx * (uint32_t)by + keybase;
       p.GP[root0 + rk] = root0 + rk;
       RLt[rk++] = key;
+      atomicOr(p.bits + (key >> 5), 1u << (key & 31));
     }
   }
 

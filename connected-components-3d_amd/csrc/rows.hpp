@@ -603,6 +603,7 @@ __device__ inline void rows_tile(const LocalParams& p, const T* in, int64_t sx, 
       const uint32_t key = kbox0 + (uint32_t)bx + (uint32_t)sx * ((uint32_t)by + (uint32_t)sy * (uint32_t)bz);
       p.GP[root0 + rk] = root0 + rk;
       RLt[rk++] = key;
+      atomicOr(p.bits + (key >> 5), 1u << (key & 31));
     }
   }
 #pragma unroll 2
