@@ -219,19 +219,15 @@ __device__ inline uint32_t slot_root0(const VolGeo& v, int64_t slot) {
   return tile_root0<G>(v.sx, v.sy, v.sz, tx, ty, tz, v.key0);
 }
 
-// Lanes that share one tile in the per-tile global kernels: a tile holds a few
-// tens of requests or local roots, so a whole wave per tile would leave most
-// lanes idle.  The label pass does less work per root and runs best wider.
-constexpr int TILE_LANES = 8;
-
-// Cross-tile unions: TILE_LANES lanes per tile walk that tile's requests,
-// each (local root r, halo index h): the forest entry of r and that of the
-// local root owning halo voxel h in its own tile.
-template <class G, int QK>
+// Cross-tile unions: LANES lanes per tile walk that tile's requests, each
+// (local root r, halo index h): the forest entry of r and that of the local
+// root owning halo voxel h in its own tile.  A tile holds a few tens of
+// requests, so a whole wave per tile would leave most lanes idle.
+template <class G, int QK, int LANES>
 __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
-  const int lane = threadIdx.x & 63, sub = lane % TILE_LANES;
-  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / TILE_LANES;
-  const int64_t ngroups = ((int64_t)gridDim.x * WG) / TILE_LANES;
+  const int lane = threadIdx.x & 63, sub = lane % LANES;
+  const int64_t group = ((int64_t)blockIdx.x * WG + threadIdx.x) / LANES;
+  const int64_t ngroups = ((int64_t)gridDim.x * WG) / LANES;
   for (int64_t slot = group; slot < p.ntiles; slot += ngroups) {
     const uint32_t nq = p.TQ[slot];
     if (nq == 0) continue;
@@ -242,7 +238,7 @@ __global__ __launch_bounds__(WG) void k_requests(GlobalParams p) {
     const int64_t ty = tr / v.ntx, tx = tr - ty * v.ntx;
     const int64_t x0 = tx * G::TX, y0 = ty * G::TY, z0 = tz * G::TZ;
     const uint32_t root0 = tile_root0<G>(v.sx, v.sy, v.sz, tx, ty, tz, v.key0);
-    for (uint32_t i = sub; i < nq; i += TILE_LANES) {
+    for (uint32_t i = sub; i < nq; i += LANES) {
       const uint32_t rq = p.REQ[slot * G::HALO + i];
       const uint32_t a = root0 + (rq >> 16);
       int bx, by, bz;
@@ -1311,15 +1307,28 @@ struct cc3d_amd_batch {
 
 namespace {
 
+// k_requests over `ntiles` tiles.  Binary volumes usually hold components
+// that span most tiles, whose unions all meet at one root: there 4 lanes per
+// tile contend less than 8 (measured on 512^3 uint8 Bernoulli volumes:
+// p = 0.5 26-connected 0.42-0.47 -> 0.14-0.16 ms, p = 0.5 6-connected 0.30 ->
+// 0.21 ms, p = 0.3 26-connected 0.20-0.23 -> 0.19-0.21 ms), while the many
+// separate components of multilabel volumes keep more requests in flight with
+// 8 (512^3 uint32 Voronoi 26-connected: 0.110 ms with 8 lanes, 0.132 with 4).
+template <class G, int QK>
+void launch_requests(const GlobalParams& p, int64_t ntiles, int mode, hipStream_t s) {
+  if (mode == M_BINARY)
+    hipLaunchKernelGGL((k_requests<G, QK, 4>), dim3(grid_for(ntiles, WG / 4, 65536)), dim3(WG), 0, s, p);
+  else hipLaunchKernelGGL((k_requests<G, QK, 8>), dim3(grid_for(ntiles, WG / 8, 65536)), dim3(WG), 0, s, p);
+}
+
 template <class G>
 int run_global(cc3d_amd_job* j) {
   const GlobalParams& p = j->gp;
   hipStream_t s = j->stream;
-  const unsigned wave_blocks = grid_for(j->g.ntiles, WG / TILE_LANES, 65536);
   {
   ProfScope ps_("k_requests", s);
-  if (j->g.qk == Q_BLOCK8) hipLaunchKernelGGL((k_requests<G, Q_BLOCK8>), dim3(wave_blocks), dim3(WG), 0, s, p);
-  else hipLaunchKernelGGL((k_requests<G, Q_NONE>), dim3(wave_blocks), dim3(WG), 0, s, p);
+  if (j->g.qk == Q_BLOCK8) launch_requests<G, Q_BLOCK8>(p, j->g.ntiles, j->g.mode, s);
+  else launch_requests<G, Q_NONE>(p, j->g.ntiles, j->g.mode, s);
   }
   HIPCHECK(hipGetLastError());
   if (j->periodic && j->g.mode != M_CONT) {
@@ -2353,13 +2362,10 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   {
     ProfScope ps_("k_requests", s);
     for (const auto& tg : b->tgroups) {
-      const unsigned blocks = grid_for(tg.tiles, WG / TILE_LANES, 65536);
-      if (tg.rows()) hipLaunchKernelGGL((k_requests<GeoR, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else if (tg.fam == cc3d_amd_batch::FAM_TALL)
-        hipLaunchKernelGGL((k_requests<Geo3T, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else if (tg.fam == cc3d_amd_batch::FAM_PLANE)
-        hipLaunchKernelGGL((k_requests<Geo2, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
-      else hipLaunchKernelGGL((k_requests<Geo3, Q_NONE>), dim3(blocks), dim3(WG), 0, s, tg.gp);
+      if (tg.rows()) launch_requests<GeoR, Q_NONE>(tg.gp, tg.tiles, b->mode, s);
+      else if (tg.fam == cc3d_amd_batch::FAM_TALL) launch_requests<Geo3T, Q_NONE>(tg.gp, tg.tiles, b->mode, s);
+      else if (tg.fam == cc3d_amd_batch::FAM_PLANE) launch_requests<Geo2, Q_NONE>(tg.gp, tg.tiles, b->mode, s);
+      else launch_requests<Geo3, Q_NONE>(tg.gp, tg.tiles, b->mode, s);
     }
   }
   HIPCHECK(hipGetLastError());
