@@ -24,6 +24,38 @@ enum Quirk : int { Q_NONE = 0, Q_BLOCK8 = 1, Q_BIN4 = 2, Q_CONT8 = 3 };
 
 constexpr int WG = 256;
 
+// Wave-wide inclusive scans over the 64 lanes with DPP: shifts by 1, 2, 4 and
+// 8 lanes within each row of 16, then lane 15 of rows 0 and 2 into rows 1 and
+// 3, then lane 31 into rows 2 and 3.  A lane whose source lies outside its
+// row, or whose row the row mask leaves out, takes the identity.  No LDS
+// traffic and no per-lane address registers, unlike shuffles.  Every lane of
+// the wave must be active.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_or(uint32_t ident, uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)x, CTRL, ROWS, 0xF, false);
+}
+template <class Op>
+__device__ inline uint32_t wave_scan(uint32_t x, uint32_t ident, Op op) {
+  x = op(x, dpp_or<0x111, 0xF>(ident, x));  // row_shr:1
+  x = op(x, dpp_or<0x112, 0xF>(ident, x));  // row_shr:2
+  x = op(x, dpp_or<0x114, 0xF>(ident, x));  // row_shr:4
+  x = op(x, dpp_or<0x118, 0xF>(ident, x));  // row_shr:8
+  x = op(x, dpp_or<0x142, 0xA>(ident, x));  // row_bcast:15
+  x = op(x, dpp_or<0x143, 0xC>(ident, x));  // row_bcast:31
+  return x;
+}
+__device__ inline uint32_t wave_incl_add(uint32_t x) {
+  return wave_scan(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+__device__ inline int wave_incl_min(int x) {
+  return (int)wave_scan((uint32_t)x, 0x7FFFFFFFu, [](uint32_t a, uint32_t b) { return (int)a < (int)b ? a : b; });
+}
+__device__ inline int wave_incl_max(int x) {
+  return (int)wave_scan((uint32_t)x, 0x80000000u, [](uint32_t a, uint32_t b) { return (int)a > (int)b ? a : b; });
+}
+// Lane 63's value (the wave total of an inclusive scan), as a wave-uniform value.
+__device__ inline uint32_t wave_lane63(uint32_t x) { return (uint32_t)__builtin_amdgcn_readlane((int)x, 63); }
+
 // Device-side counters of one labelling job.
 struct Counters {
   unsigned long long epl;        // provisional-label estimate (transitions)

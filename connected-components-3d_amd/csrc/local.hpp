@@ -723,17 +723,12 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
     // queue the run starts: a wave-level scan and one LDS atomic per wave
     const uint32_t hc = (uint32_t)__popc(hneed);
-    uint32_t hincl = hc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(hincl, o);
-      if (lane >= o) hincl += y;
-    }
-    const uint32_t htot = __shfl(hincl, 63);
+    const uint32_t hincl = wave_incl_add(hc);
+    const uint32_t htot = wave_lane63(hincl);
     if (htot) {
       uint32_t hbase = 0;
       if (lane == 63) hbase = atomicAdd(&hcount, htot);
-      hbase = __shfl(hbase, 63) + hincl - hc;
+      hbase = wave_lane63(hbase) + hincl - hc;
 #pragma unroll
       for (int k = 0; k < HLEN; k++)
         if ((hneed >> k) & 1u)
@@ -854,15 +849,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
     }
     // wave-level exclusive scan of the counts, one LDS atomic per wave
     const uint32_t cnt = (uint32_t)__popc(need);
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_add(cnt);
     uint32_t base = 0;
     if (lane == 63) base = atomicAdd(&qcount, incl);
-    qs = __shfl(base, 63) + incl - cnt;
+    qs = wave_lane63(base) + incl - cnt;
   }
   if constexpr (HRUNS) {
     // unions of the halo run start prepared in step 2, then any further ones
@@ -1079,12 +1069,7 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
       c[i] = w0 + i < RW ? (uint32_t)__popc(L.w.rbits[w0 + i]) : 0u;
       own += c[i];
     }
-    uint32_t incl = own;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_add(own);
     uint32_t ex = incl - own;
 #pragma unroll
     for (int i = 0; i < WPL; i++) {
@@ -1158,15 +1143,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
   if (__any(rq != 0u)) {
     const uint32_t rc = (uint32_t)__popc(rq);
-    uint32_t rincl = rc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(rincl, o);
-      if (lane >= o) rincl += y;
-    }
+    const uint32_t rincl = wave_incl_add(rc);
     uint32_t rbase = 0;
     if (lane == 63) rbase = atomicAdd(&nreq, rincl);
-    uint32_t q = __shfl(rbase, 63) + rincl - rc;
+    uint32_t q = wave_lane63(rbase) + rincl - rc;
     uint32_t* REQt = p.REQ + slot * G::HALO;
 #pragma unroll
     for (int i = 0; i < HV; i++)
@@ -1174,14 +1154,10 @@ __device__ inline void local_tile(const LocalParams& p, int64_t tile, int64_t sl
   }
 
   // 9. counters
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    my_epl += __shfl_xor(my_epl, o);
-    my_prov += __shfl_xor(my_prov, o);
-    const int f = __shfl_xor(my_first, o), l = __shfl_xor(my_last, o);
-    my_first = f < my_first ? f : my_first;
-    my_last = l > my_last ? l : my_last;
-  }
+  my_epl = wave_lane63(wave_incl_add(my_epl));
+  my_prov = wave_lane63(wave_incl_add(my_prov));
+  my_first = (int)wave_lane63((uint32_t)wave_incl_min(my_first));
+  my_last = (int)wave_lane63((uint32_t)wave_incl_max(my_last));
   if (lane == 0) {
     L.late.red_epl[wv] = my_epl;
     L.late.red_prov[wv] = my_prov;
