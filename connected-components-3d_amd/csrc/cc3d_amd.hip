@@ -457,6 +457,62 @@ __global__ __launch_bounds__(WG) void k_scan_c(const uint32_t* bits, int64_t nwo
   }
 }
 
+// k_scan_b and k_scan_c in one launch, for up to SCAN_FUSE_MAX blocks: each
+// block adds up the (L2-resident) sums of the blocks before it instead of
+// waiting for a single-block pass over them; block 0 adds up all of them,
+// the total.
+constexpr int64_t SCAN_FUSE_MAX = 4096;
+__global__ __launch_bounds__(WG) void k_scan_bc(const uint32_t* bits, int64_t nwords, const uint32_t* bsum,
+                                                int64_t nblk, uint32_t* prefix, Counters* ctr) {
+  __shared__ uint32_t tmp[WG / 64];
+  const int64_t lim = blockIdx.x == 0 ? nblk : (int64_t)blockIdx.x;
+  uint32_t part = 0;
+  for (int64_t i = threadIdx.x; i < lim; i += WG) part += bsum[i];
+  uint32_t sum;
+  block_exclusive_scan(part, tmp, sum);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctr) ctr->N = sum;
+  const uint32_t carry = blockIdx.x == 0 ? 0u : sum;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_WPB + (int64_t)threadIdx.x * SCAN_WPT;
+  uint32_t w[SCAN_WPT];
+  load_words(bits, base, nwords, w);
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_WPT; i++) {
+    w[i] = __popc(w[i]);
+    c += w[i];
+  }
+  uint32_t total;
+  uint32_t run = carry + block_exclusive_scan(c, tmp, total);
+#pragma unroll
+  for (int i = 0; i < SCAN_WPT; i++) {
+    const uint32_t v = w[i];
+    w[i] = run;
+    run += v;
+  }
+  if (base + SCAN_WPT <= nwords) {  // 16-byte stores for whole groups
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i += 4)
+      *reinterpret_cast<uint4*>(prefix + base + i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_WPT; i++)
+      if (base + i < nwords) prefix[base + i] = w[i];
+  }
+}
+
+// Exclusive popcount prefix of nwords bitmask words (nblk = blocks of
+// SCAN_WPB words); the total goes to ctr->N when ctr is given.
+static void launch_scan(const uint32_t* bits, int64_t nwords, uint32_t* bsum, int64_t nblk, uint32_t* prefix,
+                        Counters* ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_a, dim3((unsigned)nblk), dim3(WG), 0, s, bits, nwords, bsum);
+  if (nblk <= SCAN_FUSE_MAX) {
+    hipLaunchKernelGGL(k_scan_bc, dim3((unsigned)nblk), dim3(WG), 0, s, bits, nwords, bsum, nblk, prefix, ctr);
+  } else {
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, bsum, nblk, ctr);
+    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)nblk), dim3(WG), 0, s, bits, nwords, bsum, prefix);
+  }
+}
+
 // Final label of every local root, written over its RL entry: 1 + the rank of
 // its global root's key among the roots of its volume.
 
@@ -598,9 +654,7 @@ __global__ __launch_bounds__(WG) void k_finalize_batch(GlobalParams p) {
 hipError_t scan_root_bits(const uint32_t* bits, int64_t nwords, uint32_t* bsum, uint32_t* prefix, Counters* ctr,
                           hipStream_t s) {
   const int64_t nblk = (nwords + SCAN_WPB - 1) / SCAN_WPB;
-  hipLaunchKernelGGL(k_scan_a, dim3((unsigned)nblk), dim3(WG), 0, s, bits, nwords, bsum);
-  hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, bsum, nblk, ctr);
-  hipLaunchKernelGGL(k_scan_c, dim3((unsigned)nblk), dim3(WG), 0, s, bits, nwords, bsum, prefix);
+  launch_scan(bits, nwords, bsum, nblk, prefix, ctr, s);
   return hipGetLastError();
 }
 
@@ -1365,9 +1419,7 @@ int run_global(cc3d_amd_job* j) {
   }
   {
   ProfScope ps_("k_scan", s);
-  hipLaunchKernelGGL(k_scan_a, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
-  hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)j->g.nblk, p.ctr);
-  hipLaunchKernelGGL(k_scan_c, dim3((unsigned)j->g.nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
+  launch_scan(p.bits, p.nwords, p.bsum, (int64_t)j->g.nblk, p.prefix, p.ctr, s);
   }
   HIPCHECK(hipGetLastError());
   return CC3D_AMD_OK;
@@ -1398,9 +1450,7 @@ int launch_q1(cc3d_amd_job* j) {
                      static_cast<const T*>(j->in), g.sx, g.sy, j->qbits, j->gp.ctr);
   HIPCHECK(hipGetLastError());
   if (j->stale) {
-    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)g.qnblk), dim3(WG), 0, s, j->qbits, g.qwords, j->qbsum);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, j->qbsum, g.qnblk, (Counters*)nullptr);
-    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)g.qnblk), dim3(WG), 0, s, j->qbits, g.qwords, j->qbsum, j->qprefix);
+    launch_scan(j->qbits, g.qwords, j->qbsum, g.qnblk, j->qprefix, nullptr, s);
     HIPCHECK(hipGetLastError());
   }
   return CC3D_AMD_OK;
@@ -1456,9 +1506,7 @@ hipError_t batch_q1(const cc3d_amd_batch* b, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_words, dim3(grid_for(st.qwords, WG, 8192)), dim3(WG), 0, s, qb, st.qwords);
     hipLaunchKernelGGL((k_q1_newblocks<T>), dim3(grid_for(st.qblocks, WG, 65536)), dim3(WG), 0, s,
                        static_cast<const T*>(v.in), v.sx, v.sy, qb, b->gp.ctr + 1 + st.k);
-    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)st.qnblk), dim3(WG), 0, s, qb, st.qwords, qs);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, qs, st.qnblk, (Counters*)nullptr);
-    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)st.qnblk), dim3(WG), 0, s, qb, st.qwords, qs, qp);
+    launch_scan(qb, st.qwords, qs, st.qnblk, qp, nullptr, s);
   }
   return hipGetLastError();
 }
@@ -1681,7 +1729,7 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
   if (e != hipSuccess) return fail(CC3D_AMD_ERR_HIP, std::string("k_local: ") + hipGetErrorString(e));
   {
     ProfScope ps_("k_tile_counters", s);
-    hipLaunchKernelGGL(k_tile_counters, dim3(grid_for(g.ntiles, WG * 16, 256)), dim3(WG), 0, s, lp.tcnt, g.ntiles,
+    hipLaunchKernelGGL(k_tile_counters, dim3(grid_for(g.ntiles, WG * 2, 256)), dim3(WG), 0, s, lp.tcnt, g.ntiles,
                        gp.ctr);
   }
   if (hipGetLastError() != hipSuccess) return fail(CC3D_AMD_ERR_HIP, "k_tile_counters launch failed");
@@ -2382,9 +2430,7 @@ int cc3d_amd_batch_begin(cc3d_amd_batch* b) {
   }
   {
     ProfScope ps_("k_scan", s);
-    hipLaunchKernelGGL(k_scan_a, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(WG), 0, s, p.bsum, (int64_t)b->nblk, p.ctr);
-    hipLaunchKernelGGL(k_scan_c, dim3((unsigned)b->nblk), dim3(WG), 0, s, p.bits, p.nwords, p.bsum, p.prefix);
+    launch_scan(p.bits, p.nwords, p.bsum, (int64_t)b->nblk, p.prefix, p.ctr, s);
   }
   {
     hipLaunchKernelGGL(k_batch_N, dim3((b->n + 63) / 64), dim3(64), 0, s, p.vols, b->n, p.prefix, p.nwords, p.ctr,
