@@ -52,7 +52,14 @@ __global__ void k_init(Counters* c, uint32_t* bits, int64_t nwords) {
     c->req_overflow = 0;
     c->N = 0;
   }
-  for (int64_t w = i; w < nwords; w += (int64_t)gridDim.x * blockDim.x) bits[w] = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if ((reinterpret_cast<uintptr_t>(bits) & 15) == 0) {  // 16-byte stores, then the tail
+    const int64_t n4 = nwords >> 2;
+    for (int64_t w = i; w < n4; w += stride) reinterpret_cast<uint4*>(bits)[w] = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t w = 4 * n4 + i; w < nwords; w += stride) bits[w] = 0;
+  } else {
+    for (int64_t w = i; w < nwords; w += stride) bits[w] = 0;
+  }
 }
 
 // Union tables (UTab in common.hpp), one copy per device, built once by
@@ -1669,7 +1676,7 @@ static int job_begin(cc3d_amd_job* j, const void* in, int in_dtype, int64_t sx, 
 
   {
     ProfScope ps_("k_init", s);
-    hipLaunchKernelGGL(k_init, dim3(grid_for(g.nwords, WG, 8192)), dim3(WG), 0, s, gp.ctr, gp.bits, g.nwords);
+    hipLaunchKernelGGL(k_init, dim3(grid_for((g.nwords + 3) / 4, WG, 8192)), dim3(WG), 0, s, gp.ctr, gp.bits, g.nwords);
   }
   if (hipGetLastError() != hipSuccess) return fail(CC3D_AMD_ERR_HIP, "k_init launch failed");
 
