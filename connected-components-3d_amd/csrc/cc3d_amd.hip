@@ -520,14 +520,11 @@ static void launch_scan(const uint32_t* bits, int64_t nwords, uint32_t* bsum, in
   }
 }
 
-// Final label of every local root, written over its RL entry: 1 + the rank of
-// its global root's key among the roots of its volume.
-
 // Stage 2: every voxel of the tile gets its local root's label.  Thread t
 // owns the column the local kernel gave it: its 8 root indices are one 8- or
-// 16-byte load, issued together with a coalesced copy of the tile's label
-// table into LDS, so the per-voxel lookups hit LDS instead of scattered
-// global reads.
+// 16-byte load, issued before the tile's label table is built in LDS (one
+// entry per local root, from its global root's key), so the per-voxel
+// lookups hit LDS instead of scattered global reads.
 // Final label of a local root whose global root has key r: 1 + the rank of r
 // among the global roots' keys, the bits left set in the bitmask (prefix: the
 // bitmask's exclusive popcount prefix per word).  base: the prefix at the
